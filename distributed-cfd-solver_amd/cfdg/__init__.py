@@ -1,0 +1,354 @@
+"""ctypes binding of libcfdg.so (include/cfdg.h).
+
+The library is the product: every call below goes to compiled HIP/C++ code.
+If libcfdg.so is missing the import fails loudly -- there is no Python or CPU
+fallback for any device stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcfdg.so")
+
+# --- constants (include/cfdg.h) ---------------------------------------------
+OK = 0
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH = -1, -2, -3, -4, -5, -6
+INVISCID_FLOW, VISCOUS_FLOW = 0, 1
+FIRST_ORDER, MUSCL = 0, 1
+LIMITER = {"VENKAT": 0, "BARTH": 1, "VAN_ALBADA": 2, "UNLIMITED": 3}
+GRADIENT = {"GREEN_GAUSS_FACE": 0, "GREEN_GAUSS_CELL": 1, "LEAST_SQUARES_FACE": 2, "LEAST_SQUARES_NODE": 2}
+BC = {
+    "SUPERSONIC_INLET": 0, "SUPERSONIC_OUTLET": 1, "SUBSONIC_INLET": 2, "SUBSONIC_OUTLET": 3,
+    "SLIP_WALL": 4, "NO_SLIP_WALL": 5, "NO_SLIP_WALL_HEAT_FLUX": 6, "NO_SLIP_WALL_ADIABATIC": 6,
+    "SYMMETRY": 7, "FARFIELD": 8,
+}
+INFLOW_VELOCITY, INFLOW_MACH_ANGLES, INFLOW_MACH_DIRECTION = 0, 1, 2
+PRECOND_NONE, PRECOND_SGS = 0, 1
+SGS_MULTICOLOUR, SGS_SEQUENTIAL = 0, 1
+CONVERGED, DIVERGED, MAX_ITERATIONS = 0, 1, 2
+
+_p = C.POINTER
+_d, _i32, _i64, _u8 = C.c_double, C.c_int32, C.c_int64, C.c_uint8
+
+
+class MeshSpec(C.Structure):
+    _fields_ = [
+        ("kind", _i32), ("ni", _i32), ("nj", _i32), ("nk", _i32),
+        ("x0", _d), ("x1", _d), ("y0", _d), ("y1", _d), ("z0", _d), ("z1", _d),
+        ("y_ratio", _d), ("ramp_x", _d), ("ramp_deg", _d),
+        ("bump_h", _d), ("bump_w", _d), ("bump_x", _d),
+        ("jitter", _d), ("seed", C.c_uint64),
+        ("split_ymin", _i32), ("split_ymin_x", _d), ("reorder", _i32),
+    ]
+
+
+class MeshView(C.Structure):
+    _fields_ = [
+        ("n_cells", _i32), ("n_nodes", _i32), ("n_faces", _i32), ("n_inner_faces", _i32), ("n_patches", _i32),
+        ("cell_type", _p(_u8)), ("cell_nodes_off", _p(_i32)), ("cell_nodes", _p(_i32)),
+        ("cell_cx", _p(_d)), ("cell_cy", _p(_d)), ("cell_cz", _p(_d)), ("cell_vol", _p(_d)),
+        ("node_x", _p(_d)), ("node_y", _p(_d)), ("node_z", _p(_d)),
+        ("face_owner", _p(_i32)), ("face_neigh", _p(_i32)), ("face_patch", _p(_i32)), ("face_type", _p(_u8)),
+        ("face_nodes_off", _p(_i32)), ("face_nodes", _p(_i32)),
+        ("face_cx", _p(_d)), ("face_cy", _p(_d)), ("face_cz", _p(_d)),
+        ("face_nx", _p(_d)), ("face_ny", _p(_d)), ("face_nz", _p(_d)), ("face_area", _p(_d)),
+    ]
+
+
+class Numerics(C.Structure):
+    _fields_ = [
+        ("flow_model", _i32), ("gamma", _d), ("gas_constant", _d), ("prandtl", _d), ("prandtl_turb", _d),
+        ("reconstruction", _i32), ("limiter", _i32), ("venkat_k", _d), ("gradient", _i32), ("cfl", _d),
+    ]
+
+
+class Bc(C.Structure):
+    _fields_ = [
+        ("type", _i32), ("inflow_mode", _i32), ("p", _d), ("t", _d), ("tmp_grad", _d),
+        ("velocity", _d * 3), ("mach", _d), ("alpha_deg", _d), ("beta_deg", _d), ("direction", _d * 3),
+    ]
+
+
+class LinParams(C.Structure):
+    _fields_ = [("preconditioner", _i32), ("sgs_order", _i32), ("rel_tol", _d), ("abs_tol", _d), ("max_iter", _i32)]
+
+
+class IterResult(C.Structure):
+    _fields_ = [("status", _i32), ("iterations", _i32), ("initial_residual", _d), ("final_residual", _d)]
+
+
+class StepResult(C.Structure):
+    _fields_ = [("l2", _d * 5), ("linear", IterResult)]
+
+
+# Mesh array fields of MeshView: name -> (dtype, length attribute)
+MESH_ARRAYS = {
+    "cell_type": (np.uint8, "n_cells"), "cell_nodes_off": (np.int32, "n_cells+1"), "cell_nodes": (np.int32, None),
+    "cell_cx": (np.float64, "n_cells"), "cell_cy": (np.float64, "n_cells"), "cell_cz": (np.float64, "n_cells"),
+    "cell_vol": (np.float64, "n_cells"),
+    "node_x": (np.float64, "n_nodes"), "node_y": (np.float64, "n_nodes"), "node_z": (np.float64, "n_nodes"),
+    "face_owner": (np.int32, "n_faces"), "face_neigh": (np.int32, "n_faces"), "face_patch": (np.int32, "n_faces"),
+    "face_type": (np.uint8, "n_faces"), "face_nodes_off": (np.int32, "n_faces+1"), "face_nodes": (np.int32, None),
+    "face_cx": (np.float64, "n_faces"), "face_cy": (np.float64, "n_faces"), "face_cz": (np.float64, "n_faces"),
+    "face_nx": (np.float64, "n_faces"), "face_ny": (np.float64, "n_faces"), "face_nz": (np.float64, "n_faces"),
+    "face_area": (np.float64, "n_faces"),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libcfdg.so not built ({LIB_PATH}); run __graft_entry__.build() or make -C distributed-cfd-solver_amd")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "cfdg_mesh_generate": [vp, vp], "cfdg_mesh_from_view": [vp, vp], "cfdg_mesh_get_view": [vp, vp],
+        "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp],
+        "cfdg_create": [vp, vp, vp, _i32, vp], "cfdg_destroy": [vp], "cfdg_sizes": [vp, vp, vp, vp],
+        "cfdg_set_state": [vp, vp, _i64], "cfdg_get_state": [vp, vp, _i64], "cfdg_init_uniform": [vp, _d, vp, _d],
+        "cfdg_eval_residual": [vp], "cfdg_get_primitives": [vp, vp, _i64],
+        "cfdg_get_gradients": [vp, vp, _i64, vp, _i64], "cfdg_get_residual": [vp, vp, _i64, vp],
+        "cfdg_residual_norms": [vp, vp], "cfdg_compute_dt": [vp], "cfdg_get_dt": [vp, vp, vp],
+        "cfdg_assemble_jacobian": [vp], "cfdg_get_bsr_pattern": [vp, vp, vp, vp],
+        "cfdg_get_bsr_values": [vp, vp], "cfdg_set_bsr_values": [vp, vp],
+        "cfdg_linear_solve": [vp, vp, vp, vp, vp, vp], "cfdg_implicit_step": [vp, vp, vp],
+        "cfdg_stage_times": [vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.cfdg_mesh_free.argtypes = [vp]
+    lib.cfdg_mesh_free.restype = None
+    lib.cfdg_last_error.argtypes = [vp]
+    lib.cfdg_last_error.restype = C.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+class CfdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cfdg error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Mesh:
+    """Owns a cfdg_mesh (generated or copied from arrays)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.view = MeshView()
+        rc = lib.cfdg_mesh_get_view(self._h, C.byref(self.view))
+        if rc != OK:
+            raise CfdgError(rc, "mesh view")
+
+    @classmethod
+    def generate(cls, **kw):
+        spec = MeshSpec(y_ratio=1.0, bump_w=1.0)
+        for k, v in kw.items():
+            setattr(spec, k, v)
+        h = C.c_void_p()
+        rc = lib.cfdg_mesh_generate(C.byref(spec), C.byref(h))
+        if rc != OK:
+            raise CfdgError(rc, "mesh generation failed (see stderr)")
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, arrays, n_patches):
+        """Copy a mesh given as numpy arrays named like MeshView's fields."""
+        keep = {k: np.ascontiguousarray(arrays[k], dtype=MESH_ARRAYS[k][0]) for k in MESH_ARRAYS}
+        v = MeshView()
+        v.n_cells = len(keep["cell_vol"])
+        v.n_nodes = len(keep["node_x"])
+        v.n_faces = len(keep["face_area"])
+        v.n_inner_faces = int(np.count_nonzero(keep["face_neigh"] >= 0))
+        v.n_patches = n_patches
+        for k, a in keep.items():
+            setattr(v, k, a.ctypes.data_as(type(getattr(v, k))))
+        h = C.c_void_p()
+        rc = lib.cfdg_mesh_from_view(C.byref(v), C.byref(h))
+        if rc != OK:
+            raise CfdgError(rc, "mesh rejected (see stderr)")
+        return cls(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.cfdg_mesh_free(h)
+            self._h = None
+
+    @property
+    def n_cells(self):
+        return self.view.n_cells
+
+    @property
+    def n_faces(self):
+        return self.view.n_faces
+
+    @property
+    def n_inner_faces(self):
+        return self.view.n_inner_faces
+
+    @property
+    def n_patches(self):
+        return self.view.n_patches
+
+    def array(self, name):
+        """Copy of one mesh array as numpy."""
+        dt, ln = MESH_ARRAYS[name]
+        v = self.view
+        if name == "cell_nodes":
+            n = self.array("cell_nodes_off")[-1]
+        elif name == "face_nodes":
+            n = self.array("face_nodes_off")[-1]
+        elif ln.endswith("+1"):
+            n = getattr(v, ln[:-2]) + 1
+        else:
+            n = getattr(v, ln)
+        ptr = getattr(v, name)
+        return np.ctypeslib.as_array(ptr, shape=(int(n),)).astype(dt, copy=True)
+
+    def arrays(self):
+        return {k: self.array(k) for k in MESH_ARRAYS}
+
+    def patch_names(self):
+        out = []
+        buf = C.create_string_buffer(128)
+        for p in range(self.n_patches):
+            lib.cfdg_mesh_patch_name(self._h, p, buf, 128)
+            out.append(buf.value.decode())
+        return out
+
+    def reorder_rcm(self):
+        rc = lib.cfdg_mesh_reorder_rcm(self._h)
+        if rc != OK:
+            raise CfdgError(rc, "reorder failed")
+        lib.cfdg_mesh_get_view(self._h, C.byref(self.view))
+
+
+class Solver:
+    """One device context: state, residual, Jacobian, linear solver."""
+
+    def __init__(self, mesh, numerics, bcs):
+        self.mesh = mesh
+        self.numerics = numerics
+        arr = (Bc * len(bcs))(*bcs)
+        h = C.c_void_p()
+        rc = lib.cfdg_create(mesh._h, C.byref(numerics), arr, len(bcs), C.byref(h))
+        if rc != OK:
+            raise CfdgError(rc, "cfdg_create failed (see stderr)")
+        self._h = h
+        nc, nt, nz = _i32(), _i32(), _i64()
+        lib.cfdg_sizes(h, C.byref(nc), C.byref(nt), C.byref(nz))
+        self.n_cells, self.n_total, self.nnzb = nc.value, nt.value, nz.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cfdg_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise CfdgError(rc, lib.cfdg_last_error(self._h).decode())
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (5, self.n_cells)
+        self._chk(lib.cfdg_set_state(self._h, _ptr(U), self.n_cells))
+
+    def get_state(self):
+        U = np.empty((5, self.n_cells))
+        self._chk(lib.cfdg_get_state(self._h, _ptr(U), self.n_cells))
+        return U
+
+    def init_uniform(self, rho, velocity, p):
+        vel = (_d * 3)(*velocity)
+        self._chk(lib.cfdg_init_uniform(self._h, rho, vel, p))
+
+    def eval_residual(self):
+        self._chk(lib.cfdg_eval_residual(self._h))
+
+    def primitives(self):
+        q = np.empty((5, self.n_total))
+        self._chk(lib.cfdg_get_primitives(self._h, _ptr(q), self.n_total))
+        return q
+
+    def gradients(self):
+        g = np.empty((15, self.n_total))
+        phi = np.empty((5, self.n_cells))
+        self._chk(lib.cfdg_get_gradients(self._h, _ptr(g), self.n_total, _ptr(phi), self.n_cells))
+        return g, phi
+
+    def residual(self):
+        R = np.empty((5, self.n_cells))
+        lam = np.empty(self.n_cells)
+        self._chk(lib.cfdg_get_residual(self._h, _ptr(R), self.n_cells, _ptr(lam)))
+        return R, lam
+
+    def residual_norms(self):
+        l2 = np.empty(5)
+        self._chk(lib.cfdg_residual_norms(self._h, _ptr(l2)))
+        return l2
+
+    def compute_dt(self):
+        self._chk(lib.cfdg_compute_dt(self._h))
+
+    def dt(self):
+        dt = np.empty(self.n_cells)
+        al = np.empty(self.n_cells)
+        self._chk(lib.cfdg_get_dt(self._h, _ptr(dt), _ptr(al)))
+        return dt, al
+
+    def assemble_jacobian(self):
+        self._chk(lib.cfdg_assemble_jacobian(self._h))
+
+    def bsr_pattern(self):
+        rp = np.empty(self.n_cells + 1, np.int32)
+        cols = np.empty(self.nnzb, np.int32)
+        dg = np.empty(self.n_cells, np.int32)
+        self._chk(lib.cfdg_get_bsr_pattern(self._h, _ptr(rp), _ptr(cols), _ptr(dg)))
+        return rp, cols, dg
+
+    def bsr_values(self):
+        v = np.empty((self.nnzb, 5, 5))
+        self._chk(lib.cfdg_get_bsr_values(self._h, _ptr(v)))
+        return v
+
+    def set_bsr_values(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.nnzb * 25
+        self._chk(lib.cfdg_set_bsr_values(self._h, _ptr(v)))
+
+    def linear_solve(self, lp, b=None, x0=None):
+        x = np.empty(5 * self.n_cells)
+        res = IterResult()
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        xx = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        self._chk(lib.cfdg_linear_solve(self._h, C.byref(lp), _ptr(bb), _ptr(xx), _ptr(x), C.byref(res)))
+        return x, res
+
+    def implicit_step(self, lp):
+        out = StepResult()
+        self._chk(lib.cfdg_implicit_step(self._h, C.byref(lp), C.byref(out)))
+        return out
+
+    def stage_times(self):
+        ms = np.empty(6)
+        self._chk(lib.cfdg_stage_times(self._h, _ptr(ms)))
+        return ms
+
+    def time_flux_kernel(self, reps):
+        ms = _d()
+        self._chk(lib.cfdg_time_flux_kernel(self._h, reps, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,113 @@
+/*
+ * oracle.h -- CPU restatement of the reference solver's single-rank implicit
+ * iteration.  TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the
+ * CPU-baseline leg of the benchmark link it as the checker; the product never
+ * does.
+ *
+ * The structs below have exactly the layout of their cfdg.h counterparts so a
+ * test can hand the same argument blocks to both libraries.
+ */
+#ifndef CFD_ORACLE_H
+#define CFD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t n_cells, n_nodes, n_faces, n_inner_faces, n_patches;
+    const uint8_t* cell_type;
+    const int32_t* cell_nodes_off;
+    const int32_t* cell_nodes;
+    const double* cell_cx; const double* cell_cy; const double* cell_cz; const double* cell_vol;
+    const double* node_x; const double* node_y; const double* node_z;
+    const int32_t* face_owner; const int32_t* face_neigh; const int32_t* face_patch;
+    const uint8_t* face_type;
+    const int32_t* face_nodes_off;
+    const int32_t* face_nodes;
+    const double* face_cx; const double* face_cy; const double* face_cz;
+    const double* face_nx; const double* face_ny; const double* face_nz; const double* face_area;
+} oracle_mesh_view;
+
+typedef struct {
+    int32_t flow_model;
+    double gamma, gas_constant, prandtl, prandtl_turb;
+    int32_t reconstruction, limiter;
+    double venkat_k;
+    int32_t gradient;
+    double cfl;
+} oracle_numerics;
+
+typedef struct {
+    int32_t type, inflow_mode;
+    double p, t, tmp_grad;
+    double velocity[3];
+    double mach, alpha_deg, beta_deg;
+    double direction[3];
+} oracle_bc;
+
+typedef struct {
+    int32_t preconditioner, sgs_order;
+    double rel_tol, abs_tol;
+    int32_t max_iter;
+} oracle_linparams;
+
+typedef struct {
+    int32_t status, iterations;
+    double initial_residual, final_residual;
+} oracle_iter_result;
+
+typedef struct {
+    double l2[5];
+    oracle_iter_result linear;
+} oracle_step_result;
+
+typedef struct oracle_ctx oracle_ctx;
+
+int oracle_create(const oracle_mesh_view* mesh, const oracle_numerics* num, const oracle_bc* bcs,
+                  int32_t n_bcs, oracle_ctx** out);
+int oracle_destroy(oracle_ctx* ctx);
+const char* oracle_last_error(const oracle_ctx* ctx);
+int oracle_sizes(const oracle_ctx* ctx, int32_t* n_cells, int32_t* n_total, int64_t* nnzb);
+
+int oracle_set_state(oracle_ctx* ctx, const double* U, int64_t ld);
+int oracle_get_state(oracle_ctx* ctx, double* U, int64_t ld);
+int oracle_init_uniform(oracle_ctx* ctx, double rho, const double velocity[3], double p);
+
+int oracle_eval_residual(oracle_ctx* ctx);
+int oracle_get_primitives(oracle_ctx* ctx, double* q, int64_t ld);
+int oracle_get_gradients(oracle_ctx* ctx, double* grad, int64_t ld, double* phi, int64_t ld_phi);
+int oracle_get_residual(oracle_ctx* ctx, double* R, int64_t ld, double* lam);
+int oracle_residual_norms(oracle_ctx* ctx, double l2[5]);
+int oracle_compute_dt(oracle_ctx* ctx);
+int oracle_get_dt(oracle_ctx* ctx, double* dt, double* alpha);
+
+int oracle_assemble_jacobian(oracle_ctx* ctx);
+int oracle_get_bsr_pattern(oracle_ctx* ctx, int32_t* row_ptr, int32_t* cols, int32_t* diag_idx);
+int oracle_get_bsr_values(oracle_ctx* ctx, double* values);
+int oracle_set_bsr_values(oracle_ctx* ctx, const double* values);
+
+int oracle_linear_solve(oracle_ctx* ctx, const oracle_linparams* lp, const double* b, const double* x0,
+                        double* x, oracle_iter_result* res);
+int oracle_implicit_step(oracle_ctx* ctx, const oracle_linparams* lp, oracle_step_result* out);
+
+/* Greedy colouring of the face graph in row order (colour = smallest colour
+   not used by an already-coloured lower neighbour) and the dependency levels
+   of the natural forward / backward sweeps; out arrays have n_cells entries.
+   Returns the number of colours / levels through the counters. */
+int oracle_row_groups(oracle_ctx* ctx, int32_t* colour, int32_t* n_colours, int32_t* fwd_level,
+                      int32_t* n_fwd, int32_t* bwd_level, int32_t* n_bwd);
+
+/* One SGS preconditioner application z = M^-1 r with the current matrix
+   (sgs_order selects sequential or multicolour row order). */
+int oracle_sgs_apply(oracle_ctx* ctx, int32_t sgs_order, const double* r, double* z);
+/* y = A x (AoS block vectors) */
+int oracle_spmv(oracle_ctx* ctx, const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
