@@ -115,7 +115,7 @@ def _load():
         "cfdg_assemble_jacobian": [vp], "cfdg_get_bsr_pattern": [vp, vp, vp, vp],
         "cfdg_get_bsr_values": [vp, vp], "cfdg_set_bsr_values": [vp, vp],
         "cfdg_linear_solve": [vp, vp, vp, vp, vp, vp], "cfdg_implicit_step": [vp, vp, vp],
-        "cfdg_stage_times": [vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
+        "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -337,6 +337,18 @@ class Solver:
         xx = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         self._chk(lib.cfdg_linear_solve(self._h, C.byref(lp), _ptr(bb), _ptr(xx), _ptr(x), C.byref(res)))
         return x, res
+
+    def sgs_apply(self, order, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        self._chk(lib.cfdg_sgs_apply(self._h, order, _ptr(r), _ptr(z)))
+        return z
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._chk(lib.cfdg_spmv(self._h, _ptr(x), _ptr(y)))
+        return y
 
     def implicit_step(self, lp):
         out = StepResult()

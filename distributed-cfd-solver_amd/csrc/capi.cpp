@@ -1011,6 +1011,36 @@ int cfdg_linear_solve(cfdg_ctx* c, const cfdg_linparams* lp, const double* b, co
     });
 }
 
+int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* z) {
+    return guarded(c, [&] {
+        if (!r || !z) throw CfdgError(CFDG_ERR_ARG, "null vector");
+        if (sgs_order != CFDG_SGS_MULTICOLOUR && sgs_order != CFDG_SGS_SEQUENTIAL)
+            throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
+        const std::size_t bytes = 5ULL * static_cast<std::size_t>(c->nc) * sizeof(double);
+        Precond P;
+        P.kind = CFDG_PRECOND_SGS;
+        P.order = sgs_order;
+        precond_setup(*c, P);
+        hip_check(hipMemcpyAsync(c->s.get(), r, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+        precond_apply(*c, P, c->s.get(), c->sh.get());
+        hip_check(hipMemcpyAsync(z, c->sh.get(), bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+        c->check_launch();
+        c->sync();
+    });
+}
+
+int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
+    return guarded(c, [&] {
+        if (!x || !y) throw CfdgError(CFDG_ERR_ARG, "null vector");
+        const std::size_t bytes = 5ULL * static_cast<std::size_t>(c->nc) * sizeof(double);
+        hip_check(hipMemcpyAsync(c->s.get(), x, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+        cfdg::launch_spmv(c->bargs(), c->values.get(), c->s.get(), c->t.get(), c->stream);
+        hip_check(hipMemcpyAsync(y, c->t.get(), bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+        c->check_launch();
+        c->sync();
+    });
+}
+
 int cfdg_implicit_step(cfdg_ctx* c, const cfdg_linparams* lp, cfdg_step_result* out) {
     return guarded(c, [&] {
         if (!lp) throw CfdgError(CFDG_ERR_ARG, "null linear parameters");

@@ -39,6 +39,8 @@
  *   cfdg_linear_solve      BSRLinearSystem::solve = SgsPreconditioner::setup +
  *                          BiCGSTAB::solve (bsr_linear_system.hpp:93-97,
  *                          src/linalg/bicgstab.cpp:42-201)
+ *   cfdg_sgs_apply         SgsPreconditioner::{setup, apply} (preconditioners.cpp:62-194)
+ *   cfdg_spmv              BsrMatrix::apply (bsr_matrix.cpp:10-20)
  *   cfdg_implicit_step     BackwardEuler::advance (implicit_euler.hpp:34-76)
  */
 #ifndef CFDG_H
@@ -225,6 +227,13 @@ typedef struct {
    the solution when non-NULL. */
 int cfdg_linear_solve(cfdg_ctx* ctx, const cfdg_linparams* lp, const double* b, const double* x0,
                       double* x, cfdg_iter_result* res);
+
+/* z = M^-1 r for one SGS application with the current BSR values: z = 0, a
+   forward and a backward sweep (SgsPreconditioner::apply, preconditioners.cpp:
+   171-194; LU of the diagonal blocks as in :62-93).  AoS vectors, 5 n_cells. */
+int cfdg_sgs_apply(cfdg_ctx* ctx, int32_t sgs_order, const double* r, double* z);
+/* y = A x with the current BSR values (BsrMatrix::apply, bsr_matrix.cpp:10-20). */
+int cfdg_spmv(cfdg_ctx* ctx, const double* x, double* y);
 
 typedef struct {
     double l2[5];                 /* residual norms of R(U^n) */

@@ -406,6 +406,7 @@ struct oracle_ctx {
     std::vector<double> dvx, dvy, dvz, dinv;
     // fields (SoA, n_total)
     std::vector<double> U, q, gx, gy, gz, phi, res, lam, dt, alpha;
+    std::vector<double> fscale;  // sum over the cell's faces of |F|, per variable
     bool have_residual = false;
     // BSR
     std::vector<i32> row_ptr, cols, diag;
@@ -1002,6 +1003,7 @@ void compute_limiters(oracle_ctx& c) {
 void compute_fluxes(oracle_ctx& c) {
     for (int v = 0; v < 5; ++v) std::fill_n(plane(c.res, c.ntot, v), c.ntot, 0.0);
     std::fill(c.lam.begin(), c.lam.end(), 0.0);
+    std::fill(c.fscale.begin(), c.fscale.end(), 0.0);
     const bool muscl = c.num.reconstruction == 1;
     const bool visc = c.num.flow_model == 1;
     Views vv;
@@ -1050,9 +1052,11 @@ void compute_fluxes(oracle_ctx& c) {
         }
         for (int v = 0; v < 5; ++v) plane(c.res, c.ntot, v)[c0] += F[v];
         c.lam[c0] += lam_f;
+        for (int v = 0; v < 5; ++v) plane(c.fscale, c.ntot, v)[c0] += std::fabs(F[v]);
         if (inner) {
             for (int v = 0; v < 5; ++v) plane(c.res, c.ntot, v)[c1] -= F[v];
             c.lam[c1] += lam_f;
+            for (int v = 0; v < 5; ++v) plane(c.fscale, c.ntot, v)[c1] += std::fabs(F[v]);
         }
     }
 }
@@ -1453,6 +1457,7 @@ int oracle_create(const oracle_mesh_view* m, const oracle_numerics* num, const o
         c->gz.assign(5 * nt, 0.0);
         c->phi.assign(5 * nt, 0.0);
         c->res.assign(5 * nt, 0.0);
+        c->fscale.assign(5 * nt, 0.0);
         c->lam.assign(nc, 0.0);
         c->dt.assign(nc, 0.0);
         c->alpha.assign(nc, 0.0);
@@ -1534,6 +1539,12 @@ int oracle_get_residual(oracle_ctx* c, double* R, int64_t ld, double* lam) {
     if (R)
         for (int v = 0; v < 5; ++v) std::copy_n(plane(c->res, c->ntot, v), c->nc, R + v * ld);
     if (lam) std::copy_n(c->lam.data(), c->nc, lam);
+    return 0;
+}
+
+int oracle_get_flux_scale(oracle_ctx* c, double* S, int64_t ld) {
+    if (!c || !S || ld < c->nc) return -1;
+    for (int v = 0; v < 5; ++v) std::copy_n(plane(c->fscale, c->ntot, v), c->nc, S + v * ld);
     return 0;
 }
 

@@ -80,6 +80,9 @@ int oracle_eval_residual(oracle_ctx* ctx);
 int oracle_get_primitives(oracle_ctx* ctx, double* q, int64_t ld);
 int oracle_get_gradients(oracle_ctx* ctx, double* grad, int64_t ld, double* phi, int64_t ld_phi);
 int oracle_get_residual(oracle_ctx* ctx, double* R, int64_t ld, double* lam);
+/* S[v ld + c] = sum over the faces of cell c of |F_v| (the magnitude scale of
+   the residual sum, used to state per-cell tolerances) */
+int oracle_get_flux_scale(oracle_ctx* ctx, double* S, int64_t ld);
 int oracle_residual_norms(oracle_ctx* ctx, double l2[5]);
 int oracle_compute_dt(oracle_ctx* ctx);
 int oracle_get_dt(oracle_ctx* ctx, double* dt, double* alpha);

@@ -31,7 +31,7 @@ def _load():
         "oracle_set_state": [vp, vp, i64], "oracle_get_state": [vp, vp, i64], "oracle_init_uniform": [vp, d, vp, d],
         "oracle_eval_residual": [vp], "oracle_get_primitives": [vp, vp, i64],
         "oracle_get_gradients": [vp, vp, i64, vp, i64], "oracle_get_residual": [vp, vp, i64, vp],
-        "oracle_residual_norms": [vp, vp], "oracle_compute_dt": [vp], "oracle_get_dt": [vp, vp, vp],
+        "oracle_residual_norms": [vp, vp], "oracle_get_flux_scale": [vp, vp, i64], "oracle_compute_dt": [vp], "oracle_get_dt": [vp, vp, vp],
         "oracle_assemble_jacobian": [vp], "oracle_get_bsr_pattern": [vp, vp, vp, vp],
         "oracle_get_bsr_values": [vp, vp], "oracle_set_bsr_values": [vp, vp],
         "oracle_linear_solve": [vp, vp, vp, vp, vp, vp], "oracle_implicit_step": [vp, vp, vp],
@@ -116,6 +116,11 @@ class Oracle:
         lam = np.empty(self.n_cells)
         self._chk(lib.oracle_get_residual(self._h, _ptr(R), self.n_cells, _ptr(lam)))
         return R, lam
+
+    def flux_scale(self):
+        S = np.empty((5, self.n_cells))
+        self._chk(lib.oracle_get_flux_scale(self._h, _ptr(S), self.n_cells))
+        return S
 
     def residual_norms(self):
         l2 = np.empty(5)
