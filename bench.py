@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark: flux-residual throughput (Mcells/s) of the HIP path on one MI355X.
+
+Workload (one "step"): one full residual evaluation -- conservative->primitive,
+boundary ghosts, node-stencil least-squares gradients + Venkatakrishnan
+limiter, ghost gradients, and the HLLC/MUSCL + viscous flux residual -- on the
+laminar flat-plate mesh: 256 x 128 x 64 hexahedra (2,097,152 cells), wall-normal
+growth 1.08, +-10 % seeded interior jitter, reverse Cuthill-McKee order,
+Mach 0.3, Re_L = 1e5, FP64.  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  Extra fields: the flux kernel's roofline, the per-stage
+device times, one implicit step (Jacobian + BiCGSTAB/SGS) and the CPU baseline
+(the reference's own compiled code where available, else the C++ oracle port).
+
+usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "distributed-cfd-solver_amd"))
+
+METRIC = "Mcells/s flux residual; time to 1e-8 residual drop: plain vs mixed+learned"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", default="256x128x64", help="plate mesh ni x nj x nk")
+    ap.add_argument("--flux-reps", type=int, default=20)
+    ap.add_argument("--implicit-steps", type=int, default=2)
+    ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "off"], default="auto")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="profiling mode: one 1 GiB device copy (counter calibration) and flux-kernel launches only")
+    return ap.parse_args()
+
+
+def flux_algorithmic_bytes(nc, nbf, nf, viscous=True):
+    # per evaluation: cells 272 B (q 40, grad 120, phi 40, centroid 24, R 40, lambda 8),
+    # boundary ghosts g B, faces f B (owner+neigh 8, normal 24, area 8, centroid 24 [+ d 24, 1/d 8])
+    g, f = (160, 96) if viscous else (40, 64)
+    return 272 * nc + g * nbf + f * nf
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (requested --gpus {a.gpus}); "
+              "the multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    ni, nj, nk = (int(x) for x in a.size.split("x"))
+    case = cases.plate(ni=ni, nj=nj, nk=nk)
+    t0 = time.time()
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    t_mesh = time.time() - t0
+    t0 = time.time()
+    solver = cfdg.Solver(mesh, case.numerics, case.bcs)
+    t_setup = time.time() - t0
+    nc, nf, ninner = mesh.n_cells, mesh.n_faces, mesh.n_inner_faces
+    nbf = nf - ninner
+
+    solver.init_uniform(case.rho, case.velocity, case.p)
+    U = solver.get_state()
+    U = U * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal(U.shape))
+    solver.set_state(U)
+
+    if a.pmc_run:
+        # counter calibration: one device-to-device copy of exactly 1 GiB (the HIP
+        # runtime's copy kernel), then the residual stages and flux launches
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = 1 << 30
+        src, dst = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+        assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        solver.eval_residual()
+        solver.time_flux_kernel(5)
+        print(json.dumps({"pmc_run": True, "copy_bytes": nbytes, "flux_launches": 5 + 1}))
+        return
+
+    for _ in range(a.warmup):
+        solver.eval_residual()
+    stage_acc = np.zeros(6)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        solver.eval_residual()
+        stage_acc += solver.stage_times()
+    t_total = time.perf_counter() - t0
+    ms_per_step = 1e3 * t_total / a.steps
+    value = nc / (t_total / a.steps) / 1e6
+    stage_ms = (stage_acc / a.steps).tolist()
+
+    flux_ms = solver.time_flux_kernel(a.flux_reps)
+    B = flux_algorithmic_bytes(nc, nbf, nf, viscous=True)
+    achieved = B / (flux_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "flux_kernel_plate_2M_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                tj = json.load(f)
+            if tj.get("cells") == nc:
+                traffic = tj.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    # one full implicit step (residual, dt, Jacobian, BiCGSTAB + multicolour SGS, update)
+    lp = cases.default_linparams()
+    imp = []
+    for _ in range(a.implicit_steps):
+        t1 = time.perf_counter()
+        st = solver.implicit_step(lp)
+        imp.append({"ms": 1e3 * (time.perf_counter() - t1), "linear_iterations": st.linear.iterations,
+                    "linear_status": st.linear.status,
+                    "jacobian_ms": float(solver.stage_times()[4]), "linear_ms": float(solver.stage_times()[5])})
+
+    cpu = None
+    if a.cpu_baseline != "off":
+        cpu = cpu_baseline(a, mesh, case, U, nc)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"laminar flat plate, {ni}x{nj}x{nk} hexahedra ({nc} cells, {nf} faces), y-growth 1.08, "
+                        "10% jitter, RCM order, Mach 0.3, Re_L 1e5; one step = one full residual evaluation "
+                        "(primitives, ghosts, LSQ-node gradients + Venkatakrishnan limiter, ghost gradients, "
+                        "HLLC/MUSCL + viscous flux residual)",
+            "cells": nc, "faces": nf, "boundary_faces": nbf,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_flux (flux residual)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": B,
+            "ms_per_launch": round(flux_ms, 5),
+        },
+        "cpu_baseline": cpu,
+        "stage_ms": {"primitives_ghosts": round(stage_ms[0], 4), "gradients_limiter": round(stage_ms[1], 4),
+                     "ghost_gradients": round(stage_ms[2], 4), "flux": round(stage_ms[3], 4)},
+        "flux_kernel_mcells_s": round(nc / (flux_ms * 1e-3) / 1e6, 1),
+        "implicit_step": imp,
+        "setup_s": {"mesh": round(t_mesh, 2), "context": round(t_setup, 2)},
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(a, mesh, case, U, nc):
+    """The reference's Solver::evaluate_residual (compiled from its own sources,
+    oracle/_ref) on the same mesh and state, or the C++ oracle port."""
+    import numpy as np
+    cores = 1
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    if a.cpu_baseline in ("auto", "reference"):
+        try:
+            import ref_run
+            if ref_run.available():
+                times, _ = ref_run.time_residual(mesh.arrays(), mesh.n_patches, case.numerics, case.bcs, U,
+                                                 reps=a.cpu_reps)
+                t = float(np.median(times))
+                return {"value": round(nc / t / 1e6, 4), "unit": "Mcells/s", "cores": cores, "kind": "reference",
+                        "sample": f"{len(times)} residual evaluations of the full {nc}-cell mesh by the reference's "
+                                  "Solver::evaluate_residual, 1 MPI rank, median",
+                        "s_per_eval": round(t, 3)}
+        except Exception as e:  # noqa: BLE001  (fall back to the port and say so)
+            print(f"bench.py: reference CPU baseline unavailable ({e}); timing the oracle port", file=sys.stderr)
+        if a.cpu_baseline == "reference":
+            return None
+    from oracle_py import Oracle
+    o = Oracle(mesh, case.numerics, case.bcs)
+    o.set_state(U)
+    times = []
+    for _ in range(a.cpu_reps):
+        t0 = time.perf_counter()
+        o.eval_residual()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": round(nc / t / 1e6, 4), "unit": "Mcells/s", "cores": cores, "kind": "port",
+            "sample": f"{len(times)} residual evaluations of the full {nc}-cell mesh by the C++ oracle, median",
+            "s_per_eval": round(t, 3)}
+
+
+if __name__ == "__main__":
+    main()

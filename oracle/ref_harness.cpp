@@ -381,8 +381,48 @@ void run_solver_path(const Recs& r, const mesh::MeshPart& mp, const bc::Boundary
     }
 }
 
+// Wall time of Solver::evaluate_residual on the given state (MPI_Wtime around
+// `reps` calls), the reference's own CPU cost of one residual evaluation.
+template <typename Phys, typename Recon>
+void run_timing(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfig& bcfg, const Phys& phys, Writer& w) {
+    using EOS = sc::eos::IdealGas;
+    const EOS eos{scalar<double>(r, "gamma"), scalar<double>(r, "gas_constant")};
+    sc::SolverConfig cfg;
+    cfg.flow.gamma = eos.gamma_val;
+    cfg.flow.gas_constant = eos.R_val;
+    cfg.prandtl = scalar<double>(r, "prandtl");
+    cfg.flow_model = Phys::kHasViscous ? sc::FlowModel::ViscousFlow : sc::FlowModel::InviscidFlow;
+    cfg.gradient = grad_type(scalar<std::int32_t>(r, "gradient"));
+    cfg.limiter_venkat_k = scalar<double>(r, "venkat_k");
+    cfg.scheme = sc::TimeScheme::BackwardEuler;
+    cfg.cfl = scalar<double>(r, "cfl");
+    cfg.output_dir = "/tmp/cfdg_ref_out";
+    const double t0 = MPI_Wtime();
+    sc::Solver<EOS, sc::fluxes::HllcFlux, Recon, Phys, sc::time::BackwardEuler, solver::SteadyMode> s(
+        cfg, bcfg, eos, phys, mp, MPI_COMM_WORLD);
+    const double t_setup = MPI_Wtime() - t0;
+    const std::size_t nc = static_cast<std::size_t>(mp.n_cells);
+    const auto U = get<double>(r, "U");
+    auto us = s.u_slots();
+    for (int v = 0; v < 5; ++v)
+        for (std::size_t c = 0; c < nc; ++c) us[static_cast<std::size_t>(v)][c] = U[static_cast<std::size_t>(v) * nc + c];
+    const int reps = scalar<std::int32_t>(r, "time_reps");
+    std::vector<double> times;
+    for (int i = 0; i < reps; ++i) {
+        const double a = MPI_Wtime();
+        s.evaluate_residual(s.u_slots());
+        times.push_back(MPI_Wtime() - a);
+    }
+    w.put("time_residual", times);
+    w.put("time_setup", &t_setup, 1);
+}
+
 template <typename Phys, typename Recon>
 void run_all(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfig& bcfg, const Phys& phys, Writer& w) {
+    if (r.count("time_reps") && scalar<std::int32_t>(r, "time_reps") > 0) {
+        run_timing<Phys, Recon>(r, mp, bcfg, phys, w);
+        return;
+    }
     run_components<Phys, Recon>(r, mp, bcfg, phys, w);
     if (scalar<std::int32_t>(r, "solver_path") != 0) run_solver_path<Phys, Recon>(r, mp, bcfg, phys, w);
 }

@@ -69,3 +69,13 @@ def run(records, timeout=600):
         out = read_records(fout)
         out["_stdout"] = p.stdout
         return out
+
+
+def time_residual(mesh_arrays, n_patches, numerics, bcs, U, reps=3, timeout=900):
+    """Seconds per Solver::evaluate_residual of the reference on this mesh/state."""
+    from cfdg import LinParams
+    rec = case_records(mesh_arrays, n_patches, numerics, bcs, U, [1.0, 0.0, 0.0, 0.0, 1.0], LinParams(),
+                       steps=0, solver_path=0, rcm=0, validate=0)
+    rec["time_reps"] = np.array([reps], np.int32)
+    out = run(rec, timeout=timeout)
+    return out["time_residual"], float(out["time_setup"][0])
