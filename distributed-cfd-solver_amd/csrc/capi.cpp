@@ -1,6 +1,9 @@
-// C ABI of the library (include/cfdg.h): context set-up, device memory and
+// C ABI of the solver context (include/cfdg.h): set-up, device memory, and
 // the orchestration of the residual, Jacobian and BiCGSTAB stages on one HIP
-// stream.  Every entry point catches exceptions and turns them into status codes.
+// stream.  Every entry point catches exceptions and returns a status code.
+//
+// Internally cells live in the device (tile) order of setup.hpp; every array
+// crossing the ABI is in mesh order and is permuted here, on the host.
 #include "cfdg.h"
 
 #include <hip/hip_runtime.h>
@@ -14,21 +17,17 @@
 #include <string>
 #include <vector>
 
+#include "capi_internal.hpp"
 #include "kernels.hpp"
 #include "kernels_linalg.hpp"
 #include "mesh.hpp"
 #include "setup.hpp"
 
-struct cfdg_mesh {
-    cfdg::Mesh m;
-};
-
 namespace {
 
-struct CfdgError : std::runtime_error {
-    int code;
-    CfdgError(int c, const std::string& w) : std::runtime_error(w), code(c) {}
-};
+using cfdg::CfdgError;
+using i32 = std::int32_t;
+inline std::size_t z(long long v) { return static_cast<std::size_t>(v); }
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw CfdgError(CFDG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -66,7 +65,7 @@ class DBuf {
     std::size_t n_ = 0;
 };
 
-// Resolved boundary parameters (bc_manager.hpp:235-352 builders + resolve_inflow :37-111).
+// Resolved boundary parameters (bc_manager.hpp:235-352 builders, resolve_inflow :37-111).
 cfdg::BcDev resolve_bc(const cfdg_bc& d, double gamma, double R) {
     cfdg::BcDev b{};
     b.type = d.type;
@@ -135,36 +134,57 @@ cfdg::BcDev resolve_bc(const cfdg_bc& d, double gamma, double R) {
     return b;
 }
 
+// CSR list of mesh-numbered cells re-expressed for device rows: row r holds
+// the list of mesh cell new2old[r], entries mapped through map (or kept when
+// map is null), order unchanged.
+cfdg::Csr to_device_rows(const cfdg::Csr& in, const std::vector<i32>& new2old, const std::vector<i32>* map) {
+    cfdg::Csr out;
+    const std::size_t n = new2old.size();
+    out.off.assign(n + 1, 0);
+    for (std::size_t r = 0; r < n; ++r) {
+        const i32 c = new2old[r];
+        out.off[r + 1] = out.off[r] + (in.off[z(c) + 1] - in.off[z(c)]);
+    }
+    out.idx.resize(z(out.off.back()));
+    for (std::size_t r = 0; r < n; ++r) {
+        const i32 c = new2old[r];
+        i32 w = out.off[r];
+        for (i32 e = in.off[z(c)]; e < in.off[z(c) + 1]; ++e) out.idx[z(w++)] = map ? (*map)[z(in.idx[z(e)])] : in.idx[z(e)];
+    }
+    return out;
+}
+
 }  // namespace
 
 struct cfdg_ctx {
     std::string err;
     hipStream_t stream = nullptr;
-    int nc = 0, nf = 0, ninner = 0, nbf = 0, ntot = 0, npatch = 0;
+    int nc = 0, nf = 0, ninner = 0, nbf = 0, ntot = 0, npatch = 0, ntiles = 0;
     cfdg_numerics num{};
     cfdg::NumArgs nargs{};
     cfdg::BcTable bcs{};
+    std::vector<i32> new2old, old2new;
 
-    // mesh
-    DBuf<int> owner, neigh, cf_off, cf, ccf_off, ccf, ccn_off, ccn, bface_bc;
-    DBuf<double> fnx, fny, fnz, farea, fcx, fcy, fcz, dvx, dvy, dvz, dinv, ccx, ccy, ccz, vol, lsq;
-    std::int64_t n_edges = 0;
-    // fields
-    DBuf<double> U, q, grad, phi, res, lam, dt, alpha;
+    // mesh (device)
+    DBuf<double> frec;
+    DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
+    // fields (device order)
+    DBuf<double> U, crec, res, lam, dt, alpha;
     bool have_residual = false;
     bool have_dt = false;
-    // BSR
-    std::vector<std::int32_t> h_row_ptr, h_cols, h_diag;
+    // BSR: mesh-order pattern for the ABI, device-order pattern for the kernels
+    std::vector<i32> h_row_ptr, h_cols, h_diag;
+    std::vector<long long> blk_dev;  // mesh block k -> device block
     DBuf<int> row_ptr, cols, diag;
     DBuf<double> values, lu;
     DBuf<std::uint8_t> piv;
     DBuf<int> singular;
-    // SGS row groups: 0 multicolour fwd, 1 multicolour bwd, 2 sequential fwd, 3 sequential bwd
-    std::vector<std::int32_t> h_gofs[4];
+    // SGS row groups (device rows): 0 multicolour fwd, 1 multicolour bwd, 2 sequential fwd, 3 sequential bwd
+    std::vector<i32> h_gofs[4];
     DBuf<int> grows[4];
-    cfdg::Csr ccf_host;
+    cfdg::Csr ccf_mesh;
     bool have_seq_groups = false;
-    // Krylov work vectors (AoS, 5 nc)
+    // Krylov work vectors (AoS, 5 nc, device order)
     DBuf<double> b, x, r, r0, p, v, s, t, ph, sh;
     DBuf<double> partial, dots;
     double* h_dots = nullptr;  // pinned
@@ -177,41 +197,25 @@ struct cfdg_ctx {
         a.nc = nc;
         a.nf = nf;
         a.ninner = ninner;
-        a.ld = ntot;
-        a.owner = owner.get();
-        a.neigh = neigh.get();
-        a.fnx = fnx.get();
-        a.fny = fny.get();
-        a.fnz = fnz.get();
-        a.farea = farea.get();
-        a.fcx = fcx.get();
-        a.fcy = fcy.get();
-        a.fcz = fcz.get();
-        a.dvx = dvx.get();
-        a.dvy = dvy.get();
-        a.dvz = dvz.get();
-        a.dinv = dinv.get();
-        a.ccx = ccx.get();
-        a.ccy = ccy.get();
-        a.ccz = ccz.get();
-        a.vol = vol.get();
+        a.ntiles = ntiles;
+        a.frec = frec.get();
         a.cf_off = cf_off.get();
         a.cf = cf.get();
+        a.cf_slot = cf_slot.get();
         a.ccf_off = ccf_off.get();
         a.ccf = ccf.get();
         a.ccn_off = ccn_off.get();
         a.ccn = ccn.get();
-        a.lsq = lsq.get();
-        a.n_edges = n_edges;
         a.bface_bc = bface_bc.get();
+        a.tile_off = tile_off.get();
+        a.tile_face_off = tile_face_off.get();
+        a.tile_faces = tile_faces.get();
         return a;
     }
     cfdg::FieldArgs fargs() const {
         cfdg::FieldArgs f{};
         f.U = U.get();
-        f.q = q.get();
-        f.grad = grad.get();
-        f.phi = phi.get();
+        f.crec = crec.get();
         f.res = res.get();
         f.lam = lam.get();
         return f;
@@ -226,6 +230,31 @@ struct cfdg_ctx {
     }
     void sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
     void check_launch() { hip_check(hipGetLastError(), "kernel launch"); }
+    template <typename T>
+    void d2h(T* dst, const T* src, std::size_t n) {
+        if (n) hip_check(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream), "D2H");
+    }
+    template <typename T>
+    void h2d(T* dst, const T* src, std::size_t n) {
+        if (n) hip_check(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream), "H2D");
+    }
+    // AoS block vector (5 per cell): mesh order <-> device order
+    std::vector<double> vec_to_device(const double* x_mesh) const {
+        std::vector<double> d(z(nc) * 5);
+        for (int r = 0; r < nc; ++r)
+            for (int k = 0; k < 5; ++k) d[z(r) * 5 + z(k)] = x_mesh[z(new2old[z(r)]) * 5 + z(k)];
+        return d;
+    }
+    void vec_to_mesh(const std::vector<double>& d, double* x_mesh) const {
+        for (int r = 0; r < nc; ++r)
+            for (int k = 0; k < 5; ++k) x_mesh[z(new2old[z(r)]) * 5 + z(k)] = d[z(r) * 5 + z(k)];
+    }
+    std::vector<double> download_records() {
+        std::vector<double> h(z(ntot) * cfdg::kRec);
+        d2h(h.data(), crec.get(), h.size());
+        sync();
+        return h;
+    }
 
     ~cfdg_ctx() {
         for (auto& e : ev)
@@ -252,17 +281,21 @@ int guarded(cfdg_ctx* c, F&& fn) {
     }
 }
 
+void upload_groups(cfdg_ctx& c, int slot, const std::vector<i32>& label_mesh, i32 n_labels, bool reverse) {
+    const cfdg::RowGroups g = cfdg::groups_from_labels(label_mesh, n_labels, reverse);
+    std::vector<i32> rows(g.rows.size());
+    for (std::size_t i = 0; i < rows.size(); ++i) rows[i] = c.old2new[z(g.rows[i])];
+    c.h_gofs[slot] = g.off;
+    c.grows[slot].upload(rows, c.stream);
+}
+
 void build_seq_groups(cfdg_ctx& c) {
     if (c.have_seq_groups) return;
-    std::int32_t nf = 0, nb = 0;
-    const auto lf = cfdg::forward_levels(c.ccf_host, c.nc, nf);
-    const auto lb = cfdg::backward_levels(c.ccf_host, c.nc, nb);
-    const auto gf = cfdg::groups_from_labels(lf, nf, false);
-    const auto gb = cfdg::groups_from_labels(lb, nb, false);
-    c.h_gofs[2] = gf.off;
-    c.h_gofs[3] = gb.off;
-    c.grows[2].upload(gf.rows, c.stream);
-    c.grows[3].upload(gb.rows, c.stream);
+    i32 nfw = 0, nbw = 0;
+    const auto lf = cfdg::forward_levels(c.ccf_mesh, c.nc, nfw);
+    const auto lb = cfdg::backward_levels(c.ccf_mesh, c.nc, nbw);
+    upload_groups(c, 2, lf, nfw, false);
+    upload_groups(c, 3, lb, nbw, false);
     c.sync();
     c.have_seq_groups = true;
 }
@@ -288,20 +321,16 @@ void eval_residual(cfdg_ctx& c) {
     c.check_launch();
     c.sync();
     float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]), "elapsed");
-    c.stage_ms[0] = ms;
-    hip_check(hipEventElapsedTime(&ms, c.ev[1], c.ev[2]), "elapsed");
-    c.stage_ms[1] = ms;
-    hip_check(hipEventElapsedTime(&ms, c.ev[2], c.ev[3]), "elapsed");
-    c.stage_ms[2] = ms;
-    hip_check(hipEventElapsedTime(&ms, c.ev[3], c.ev[4]), "elapsed");
-    c.stage_ms[3] = ms;
+    for (int k = 0; k < 4; ++k) {
+        hip_check(hipEventElapsedTime(&ms, c.ev[k], c.ev[k + 1]), "elapsed");
+        c.stage_ms[k] = ms;
+    }
     c.have_residual = true;
 }
 
 void compute_dt(cfdg_ctx& c) {
     if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "compute_dt before eval_residual");
-    cfdg::launch_dt(c.margs(), c.lam.get(), c.num.cfl, c.dt.get(), c.alpha.get(), c.stream);
+    cfdg::launch_dt(c.margs(), c.fargs(), c.lam.get(), c.num.cfl, c.dt.get(), c.alpha.get(), c.stream);
     c.check_launch();
     c.sync();
     c.have_dt = true;
@@ -320,9 +349,7 @@ void assemble_jacobian(cfdg_ctx& c) {
 }
 
 void fetch_dots(cfdg_ctx& c, int n) {
-    hip_check(hipMemcpyAsync(c.h_dots, c.dots.get(), static_cast<std::size_t>(n) * sizeof(double), hipMemcpyDeviceToHost,
-                             c.stream),
-              "dots D2H");
+    c.d2h(c.h_dots, c.dots.get(), static_cast<std::size_t>(n));
     c.sync();
 }
 
@@ -354,10 +381,9 @@ void precond_apply(cfdg_ctx& c, const Precond& P, const double* r, double* z) {
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int gi = gf + sweep;
         const auto& off = c.h_gofs[gi];
-        for (std::size_t k = 0; k + 1 < off.size(); ++k) {
+        for (std::size_t k = 0; k + 1 < off.size(); ++k)
             cfdg::launch_sgs_group(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.lu.get(),
                                    c.piv.get(), r, z, c.stream);
-        }
     }
 }
 
@@ -368,9 +394,11 @@ void precond_setup(cfdg_ctx& c, const Precond& P) {
     cfdg::launch_lu_factor(c.nc, c.diag.get(), c.values.get(), c.lu.get(), c.piv.get(), c.singular.get(), c.stream);
     c.check_launch();
     int sing = 0;
-    hip_check(hipMemcpyAsync(&sing, c.singular.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream), "D2H");
+    c.d2h(&sing, c.singular.get(), 1);
     c.sync();
-    if (sing != 0) throw CfdgError(CFDG_ERR_SINGULAR, "SGS: singular diagonal block at row " + std::to_string(sing - 1));
+    if (sing != 0)
+        throw CfdgError(CFDG_ERR_SINGULAR,
+                        "SGS: singular diagonal block at row " + std::to_string(c.new2old[z(sing - 1)]));
 }
 
 // Right-preconditioned BiCGSTAB (bicgstab.cpp:42-201) on device vectors; x holds
@@ -488,15 +516,19 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
     if (lp.sgs_order != CFDG_SGS_MULTICOLOUR && lp.sgs_order != CFDG_SGS_SEQUENTIAL)
         throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
     if (lp.max_iter < 0) throw CfdgError(CFDG_ERR_ARG, "max_iter must be >= 0");
-    const std::size_t bytes = 5ULL * static_cast<std::size_t>(c.nc) * sizeof(double);
+    const std::size_t n5 = 5ULL * static_cast<std::size_t>(c.nc);
     if (hb) {
-        hip_check(hipMemcpyAsync(c.b.get(), hb, bytes, hipMemcpyHostToDevice, c.stream), "b H2D");
+        const auto d = c.vec_to_device(hb);
+        c.h2d(c.b.get(), d.data(), n5);
+        c.sync();
     } else {
         if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "implicit right-hand side needs a residual");
-        cfdg::launch_rhs(c.nc, c.ntot, c.res.get(), c.b.get(), c.stream);
+        cfdg::launch_rhs(c.nc, c.nc, c.res.get(), c.b.get(), c.stream);
     }
     if (hx0) {
-        hip_check(hipMemcpyAsync(c.x.get(), hx0, bytes, hipMemcpyHostToDevice, c.stream), "x0 H2D");
+        const auto d = c.vec_to_device(hx0);
+        c.h2d(c.x.get(), d.data(), n5);
+        c.sync();
     } else {
         c.x.zero(c.stream);
     }
@@ -507,8 +539,13 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
     precond_setup(c, P);
     const cfdg_iter_result r = bicgstab(c, lp, P, c.x.get(), c.b.get());
     hip_check(hipEventRecord(c.ev[6], c.stream), "event");
-    if (hx) hip_check(hipMemcpyAsync(hx, c.x.get(), bytes, hipMemcpyDeviceToHost, c.stream), "x D2H");
     c.sync();
+    if (hx) {
+        std::vector<double> d(n5);
+        c.d2h(d.data(), c.x.get(), n5);
+        c.sync();
+        c.vec_to_mesh(d, hx);
+    }
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, c.ev[5], c.ev[6]), "elapsed");
     c.stage_ms[5] = ms;
@@ -517,319 +554,195 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
 
 void residual_norms(cfdg_ctx& c, double l2[5]) {
     if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "residual norms before eval_residual");
-    cfdg::launch_res_sq(c.nc, c.ntot, c.res.get(), c.partial.get(), c.dots.get(), c.stream);
+    cfdg::launch_res_sq(c.nc, c.nc, c.res.get(), c.partial.get(), c.dots.get(), c.stream);
     fetch_dots(c, 5);
     const double scale = 1.0 / static_cast<double>(std::max(c.nc, 1));
     for (int v = 0; v < 5; ++v) l2[v] = std::sqrt(c.h_dots[v] * scale);
 }
 
-cfdg::Mesh mesh_from_view(const cfdg_mesh_view& v) {
-    if (v.n_cells <= 0 || v.n_faces <= 0 || v.n_inner_faces < 0 || v.n_inner_faces > v.n_faces || v.n_patches < 0)
-        throw CfdgError(CFDG_ERR_ARG, "mesh view: invalid sizes");
-    cfdg::Mesh m;
-    const auto nc = static_cast<std::size_t>(v.n_cells), nf = static_cast<std::size_t>(v.n_faces),
-               nn = static_cast<std::size_t>(v.n_nodes);
-    m.n_cells = v.n_cells;
-    m.n_nodes = v.n_nodes;
-    m.n_faces = v.n_faces;
-    m.n_inner_faces = v.n_inner_faces;
-    m.n_internal_faces = v.n_inner_faces;
-    m.cell_type.assign(v.cell_type, v.cell_type + nc);
-    m.cell_gid.resize(nc);
-    for (std::size_t i = 0; i < nc; ++i) m.cell_gid[i] = static_cast<std::int64_t>(i);
-    m.cell_nodes_off.assign(v.cell_nodes_off, v.cell_nodes_off + nc + 1);
-    m.cell_nodes.assign(v.cell_nodes, v.cell_nodes + m.cell_nodes_off.back());
-    m.cell_cx.assign(v.cell_cx, v.cell_cx + nc);
-    m.cell_cy.assign(v.cell_cy, v.cell_cy + nc);
-    m.cell_cz.assign(v.cell_cz, v.cell_cz + nc);
-    m.cell_vol.assign(v.cell_vol, v.cell_vol + nc);
-    m.node_x.assign(v.node_x, v.node_x + nn);
-    m.node_y.assign(v.node_y, v.node_y + nn);
-    m.node_z.assign(v.node_z, v.node_z + nn);
-    m.node_gid.resize(nn);
-    for (std::size_t i = 0; i < nn; ++i) m.node_gid[i] = static_cast<std::int64_t>(i);
-    m.face_owner.assign(v.face_owner, v.face_owner + nf);
-    m.face_neigh.assign(v.face_neigh, v.face_neigh + nf);
-    m.face_patch.assign(v.face_patch, v.face_patch + nf);
-    m.face_type.assign(v.face_type, v.face_type + nf);
-    m.face_nodes_off.assign(v.face_nodes_off, v.face_nodes_off + nf + 1);
-    m.face_nodes.assign(v.face_nodes, v.face_nodes + m.face_nodes_off.back());
-    m.face_cx.assign(v.face_cx, v.face_cx + nf);
-    m.face_cy.assign(v.face_cy, v.face_cy + nf);
-    m.face_cz.assign(v.face_cz, v.face_cz + nf);
-    m.face_nx.assign(v.face_nx, v.face_nx + nf);
-    m.face_ny.assign(v.face_ny, v.face_ny + nf);
-    m.face_nz.assign(v.face_nz, v.face_nz + nf);
-    m.face_area.assign(v.face_area, v.face_area + nf);
-    for (std::int32_t p = 0; p < v.n_patches; ++p) m.patch_names.push_back("patch" + std::to_string(p));
-    for (std::size_t i = 0; i < nc; ++i) {
-        if (m.cell_nodes_off[i + 1] < m.cell_nodes_off[i]) throw CfdgError(CFDG_ERR_MESH, "cell node offsets");
-    }
-    for (const auto n : m.cell_nodes)
-        if (n < 0 || n >= v.n_nodes) throw CfdgError(CFDG_ERR_MESH, "cell node index out of range");
-    const std::string e = cfdg::check_mesh(m);
-    if (!e.empty()) throw CfdgError(CFDG_ERR_MESH, e);
-    return m;
+void check_numerics(const cfdg_numerics* num) {
+    if (num->flow_model != CFDG_INVISCID_FLOW && num->flow_model != CFDG_VISCOUS_FLOW)
+        throw CfdgError(CFDG_ERR_CONFIG, "unknown flow model");
+    if (num->reconstruction != CFDG_FIRST_ORDER && num->reconstruction != CFDG_MUSCL)
+        throw CfdgError(CFDG_ERR_CONFIG, "unknown reconstruction");
+    if (num->limiter < CFDG_LIMITER_VENKAT || num->limiter > CFDG_LIMITER_UNLIMITED)
+        throw CfdgError(CFDG_ERR_CONFIG, "unknown limiter");
+    if (num->gradient < CFDG_GREEN_GAUSS_FACE || num->gradient > CFDG_LEAST_SQUARES_NODE)
+        throw CfdgError(CFDG_ERR_CONFIG, "unknown gradient");
+    if (!(num->gamma > 1.0) || !(num->gas_constant > 0.0) || !(num->cfl > 0.0))
+        throw CfdgError(CFDG_ERR_CONFIG, "gamma > 1, gas_constant > 0 and cfl > 0 required");
+    if (num->flow_model == CFDG_VISCOUS_FLOW && (!(num->prandtl > 0.0) || !(num->prandtl_turb > 0.0)))
+        throw CfdgError(CFDG_ERR_CONFIG, "viscous flow needs prandtl > 0 and prandtl_turb > 0");
 }
 
-thread_local std::string g_mesh_err;
+void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, const cfdg_bc* bcs, int32_t n_bcs) {
+    if (n_bcs != static_cast<int32_t>(m.patch_names.size()))
+        throw CfdgError(CFDG_ERR_ARG, "one boundary condition per mesh patch is required");
+    if (n_bcs > cfdg::kMaxPatches) throw CfdgError(CFDG_ERR_ARG, "too many patches");
+    check_numerics(num);
+
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (ndev < 1) throw CfdgError(CFDG_ERR_HIP, "no HIP device");
+    hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : c.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c.h_dots), 8 * sizeof(double)), "hipHostMalloc");
+
+    c.num = *num;
+    c.nc = m.n_cells;
+    c.nf = m.n_faces;
+    c.ninner = m.n_inner_faces;
+    c.nbf = m.n_faces - m.n_inner_faces;
+    c.ntot = c.nc + c.nbf;
+    c.npatch = n_bcs;
+    c.nargs.gas = cfdg::Gas{num->gamma, num->gas_constant, num->prandtl, num->prandtl_turb};
+    c.nargs.viscous = num->flow_model == CFDG_VISCOUS_FLOW;
+    c.nargs.muscl = num->reconstruction == CFDG_MUSCL;
+    c.nargs.limiter = num->limiter;
+    c.nargs.gradient = num->gradient;
+    c.nargs.k3 = num->venkat_k * num->venkat_k * num->venkat_k;
+    for (int32_t p = 0; p < n_bcs; ++p) c.bcs.bc[p] = resolve_bc(bcs[p], num->gamma, num->gas_constant);
+    hipStream_t s = c.stream;
+
+    // mesh-numbered connectivity, device tiling and permutation
+    const cfdg::Csr cf_mesh = cfdg::cell_faces(m);
+    c.ccf_mesh = cfdg::cell_cells_by_face(m);
+    const cfdg::Tiling tl = cfdg::tile_cells(m, c.ccf_mesh, cf_mesh, cfdg::kTileCells, cfdg::kTileFaces);
+    c.new2old = tl.new2old;
+    c.old2new = tl.old2new;
+    c.ntiles = static_cast<int>(tl.tile_off.size()) - 1;
+
+    // face records in device numbering
+    std::vector<i32> owner_dev(z(c.nf)), neigh_dev(z(c.nf));
+    {
+        std::vector<double> dx, dy, dz, di;
+        cfdg::face_distances(m, dx, dy, dz, di);
+        std::vector<double> fr(z(c.nf) * cfdg::kFRec, 0.0);
+        for (int f = 0; f < c.nf; ++f) {
+            double* r = fr.data() + z(f) * cfdg::kFRec;
+            r[cfdg::kFN] = m.face_nx[z(f)];
+            r[cfdg::kFN + 1] = m.face_ny[z(f)];
+            r[cfdg::kFN + 2] = m.face_nz[z(f)];
+            r[cfdg::kFA] = m.face_area[z(f)];
+            r[cfdg::kFC] = m.face_cx[z(f)];
+            r[cfdg::kFC + 1] = m.face_cy[z(f)];
+            r[cfdg::kFC + 2] = m.face_cz[z(f)];
+            r[cfdg::kFD] = dx[z(f)];
+            r[cfdg::kFD + 1] = dy[z(f)];
+            r[cfdg::kFD + 2] = dz[z(f)];
+            r[cfdg::kFDinv] = di[z(f)];
+            owner_dev[z(f)] = c.old2new[z(m.face_owner[z(f)])];
+            neigh_dev[z(f)] = (m.face_neigh[z(f)] >= 0) ? c.old2new[z(m.face_neigh[z(f)])] : -1;
+            i32 cells[2] = {owner_dev[z(f)], neigh_dev[z(f)]};
+            std::memcpy(r + cfdg::kFCells, cells, sizeof(cells));
+        }
+        c.frec.upload(fr, s);
+        c.sync();
+    }
+    {
+        const cfdg::Csr cf_dev = to_device_rows(cf_mesh, c.new2old, nullptr);
+        const cfdg::TileFaces tf = cfdg::tile_faces(tl, cf_dev, owner_dev);
+        for (int k = 0; k < c.ntiles; ++k) {
+            if (tf.tile_face_off[z(k) + 1] - tf.tile_face_off[z(k)] > cfdg::kTileFaces ||
+                tl.tile_off[z(k) + 1] - tl.tile_off[z(k)] > cfdg::kTileCells)
+                throw CfdgError(CFDG_ERR_MESH, "tile exceeds the flux kernel's LDS capacity");
+        }
+        c.cf_off.upload(cf_dev.off, s);
+        c.cf.upload(cf_dev.idx, s);
+        c.cf_slot.upload(tf.cell_slot, s);
+        c.tile_off.upload(tl.tile_off, s);
+        c.tile_face_off.upload(tf.tile_face_off, s);
+        c.tile_faces.upload(tf.faces, s);
+        const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
+        c.ccf_off.upload(ccf_dev.off, s);
+        c.ccf.upload(ccf_dev.idx, s);
+        std::vector<int> bfb(z(c.nbf));
+        for (int i = 0; i < c.nbf; ++i) bfb[z(i)] = m.face_patch[z(c.ninner + i)];
+        c.bface_bc.upload(bfb, s);
+        c.sync();
+    }
+    if (num->gradient == CFDG_LEAST_SQUARES_NODE) {
+        const cfdg::Csr st = to_device_rows(cfdg::cell_cells_by_node(m), c.new2old, &c.old2new);
+        c.ccn_off.upload(st.off, s);
+        c.ccn.upload(st.idx, s);
+        c.sync();
+    }
+    // cell records: static centroid and volume; everything else zero
+    {
+        std::vector<double> cr(z(c.ntot) * cfdg::kRec, 0.0);
+        for (int r = 0; r < c.nc; ++r) {
+            const std::size_t o = z(c.new2old[z(r)]);
+            double* p = cr.data() + z(r) * cfdg::kRec;
+            p[cfdg::kRC] = m.cell_cx[o];
+            p[cfdg::kRC + 1] = m.cell_cy[o];
+            p[cfdg::kRC + 2] = m.cell_cz[o];
+            p[cfdg::kRVol] = m.cell_vol[o];
+        }
+        c.crec.upload(cr, s);
+        c.sync();
+    }
+    // BSR: the mesh-order pattern (bsr_helpers.hpp:16-59) and its device rows
+    {
+        const cfdg::BsrPattern bp = cfdg::bsr_pattern(c.ccf_mesh, c.nc);
+        c.h_row_ptr = bp.row_ptr;
+        c.h_cols = bp.cols;
+        c.h_diag = bp.diag;
+        std::vector<i32> rp(z(c.nc) + 1, 0), cols(bp.cols.size()), dg(z(c.nc));
+        for (int r = 0; r < c.nc; ++r) {
+            const i32 o = c.new2old[z(r)];
+            rp[z(r) + 1] = rp[z(r)] + (bp.row_ptr[z(o) + 1] - bp.row_ptr[z(o)]);
+        }
+        c.blk_dev.assign(bp.cols.size(), 0);
+        for (int r = 0; r < c.nc; ++r) {
+            const i32 o = c.new2old[z(r)];
+            for (i32 k = bp.row_ptr[z(o)]; k < bp.row_ptr[z(o) + 1]; ++k) {
+                const i32 kd = rp[z(r)] + (k - bp.row_ptr[z(o)]);
+                cols[z(kd)] = c.old2new[z(bp.cols[z(k)])];
+                c.blk_dev[z(k)] = kd;
+            }
+            dg[z(r)] = rp[z(r)] + (bp.diag[z(o)] - bp.row_ptr[z(o)]);
+        }
+        c.row_ptr.upload(rp, s);
+        c.cols.upload(cols, s);
+        c.diag.upload(dg, s);
+        i32 ncol = 0;
+        const auto col = cfdg::greedy_colours(c.ccf_mesh, c.nc, ncol);
+        upload_groups(c, 0, col, ncol, false);
+        upload_groups(c, 1, col, ncol, true);
+        c.sync();
+    }
+    const std::size_t ncs = z(c.nc);
+    c.U.alloc(5 * ncs);
+    c.res.alloc(5 * ncs);
+    c.lam.alloc(ncs);
+    c.dt.alloc(ncs);
+    c.alpha.alloc(ncs);
+    c.U.zero(s);
+    c.res.zero(s);
+    c.lam.zero(s);
+    c.values.alloc(c.h_cols.size() * 25);
+    c.values.zero(s);
+    c.lu.alloc(ncs * 25);
+    c.piv.alloc(ncs * 5);
+    c.singular.alloc(1);
+    for (DBuf<double>* v : {&c.b, &c.x, &c.r, &c.r0, &c.p, &c.v, &c.s, &c.t, &c.ph, &c.sh}) {
+        v->alloc(5 * ncs);
+        v->zero(s);
+    }
+    c.partial.alloc(static_cast<std::size_t>(cfdg::kReduceBlocks) * 5);
+    c.dots.alloc(8);
+    c.sync();
+}
 
 }  // namespace
 
 extern "C" {
 
-int cfdg_mesh_generate(const cfdg_mesh_spec* spec, cfdg_mesh** out) {
-    if (!spec || !out) return CFDG_ERR_ARG;
-    *out = nullptr;
-    try {
-        cfdg::MeshSpec s;
-        s.kind = spec->kind;
-        s.ni = spec->ni;
-        s.nj = spec->nj;
-        s.nk = spec->nk;
-        s.x0 = spec->x0;
-        s.x1 = spec->x1;
-        s.y0 = spec->y0;
-        s.y1 = spec->y1;
-        s.z0 = spec->z0;
-        s.z1 = spec->z1;
-        s.y_ratio = spec->y_ratio;
-        s.ramp_x = spec->ramp_x;
-        s.ramp_deg = spec->ramp_deg;
-        s.bump_h = spec->bump_h;
-        s.bump_w = spec->bump_w;
-        s.bump_x = spec->bump_x;
-        s.jitter = spec->jitter;
-        s.seed = spec->seed;
-        s.split_ymin = spec->split_ymin;
-        s.split_ymin_x = spec->split_ymin_x;
-        s.reorder = spec->reorder;
-        auto* m = new cfdg_mesh();
-        try {
-            m->m = cfdg::generate_mesh(s);
-        } catch (...) {
-            delete m;
-            throw;
-        }
-        *out = m;
-        return CFDG_OK;
-    } catch (const std::exception& e) {
-        g_mesh_err = e.what();
-        std::fprintf(stderr, "cfdg_mesh_generate: %s\n", e.what());
-        return CFDG_ERR_MESH;
-    }
-}
-
-int cfdg_mesh_from_view(const cfdg_mesh_view* view, cfdg_mesh** out) {
-    if (!view || !out) return CFDG_ERR_ARG;
-    *out = nullptr;
-    try {
-        auto* m = new cfdg_mesh();
-        try {
-            m->m = mesh_from_view(*view);
-        } catch (...) {
-            delete m;
-            throw;
-        }
-        *out = m;
-        return CFDG_OK;
-    } catch (const CfdgError& e) {
-        std::fprintf(stderr, "cfdg_mesh_from_view: %s\n", e.what());
-        return e.code;
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "cfdg_mesh_from_view: %s\n", e.what());
-        return CFDG_ERR_MESH;
-    }
-}
-
-int cfdg_mesh_get_view(const cfdg_mesh* mesh, cfdg_mesh_view* v) {
-    if (!mesh || !v) return CFDG_ERR_ARG;
-    const cfdg::Mesh& m = mesh->m;
-    v->n_cells = m.n_cells;
-    v->n_nodes = m.n_nodes;
-    v->n_faces = m.n_faces;
-    v->n_inner_faces = m.n_inner_faces;
-    v->n_patches = static_cast<std::int32_t>(m.patch_names.size());
-    v->cell_type = m.cell_type.data();
-    v->cell_nodes_off = m.cell_nodes_off.data();
-    v->cell_nodes = m.cell_nodes.data();
-    v->cell_cx = m.cell_cx.data();
-    v->cell_cy = m.cell_cy.data();
-    v->cell_cz = m.cell_cz.data();
-    v->cell_vol = m.cell_vol.data();
-    v->node_x = m.node_x.data();
-    v->node_y = m.node_y.data();
-    v->node_z = m.node_z.data();
-    v->face_owner = m.face_owner.data();
-    v->face_neigh = m.face_neigh.data();
-    v->face_patch = m.face_patch.data();
-    v->face_type = m.face_type.data();
-    v->face_nodes_off = m.face_nodes_off.data();
-    v->face_nodes = m.face_nodes.data();
-    v->face_cx = m.face_cx.data();
-    v->face_cy = m.face_cy.data();
-    v->face_cz = m.face_cz.data();
-    v->face_nx = m.face_nx.data();
-    v->face_ny = m.face_ny.data();
-    v->face_nz = m.face_nz.data();
-    v->face_area = m.face_area.data();
-    return CFDG_OK;
-}
-
-int cfdg_mesh_patch_name(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_t buflen) {
-    if (!mesh || !buf || buflen <= 0) return CFDG_ERR_ARG;
-    if (patch < 0 || patch >= static_cast<int32_t>(mesh->m.patch_names.size())) return CFDG_ERR_ARG;
-    std::snprintf(buf, static_cast<std::size_t>(buflen), "%s", mesh->m.patch_names[static_cast<std::size_t>(patch)].c_str());
-    return CFDG_OK;
-}
-
-int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh) {
-    if (!mesh) return CFDG_ERR_ARG;
-    try {
-        cfdg::apply_cell_permutation(mesh->m, cfdg::rcm_order(mesh->m));
-        return CFDG_OK;
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "cfdg_mesh_reorder_rcm: %s\n", e.what());
-        return CFDG_ERR_MESH;
-    }
-}
-
-void cfdg_mesh_free(cfdg_mesh* mesh) { delete mesh; }
-
 int cfdg_create(const cfdg_mesh* mesh, const cfdg_numerics* num, const cfdg_bc* bcs, int32_t n_bcs, cfdg_ctx** out) {
     if (!mesh || !num || !out || (n_bcs > 0 && !bcs)) return CFDG_ERR_ARG;
     *out = nullptr;
     auto* c = new cfdg_ctx();
-    const int rc = guarded(c, [&] {
-        const cfdg::Mesh& m = mesh->m;
-        if (n_bcs != static_cast<int32_t>(m.patch_names.size()))
-            throw CfdgError(CFDG_ERR_ARG, "one boundary condition per mesh patch is required");
-        if (n_bcs > cfdg::kMaxPatches) throw CfdgError(CFDG_ERR_ARG, "too many patches");
-        if (num->flow_model != CFDG_INVISCID_FLOW && num->flow_model != CFDG_VISCOUS_FLOW)
-            throw CfdgError(CFDG_ERR_CONFIG, "unknown flow model");
-        if (num->reconstruction != CFDG_FIRST_ORDER && num->reconstruction != CFDG_MUSCL)
-            throw CfdgError(CFDG_ERR_CONFIG, "unknown reconstruction");
-        if (num->limiter < CFDG_LIMITER_VENKAT || num->limiter > CFDG_LIMITER_UNLIMITED)
-            throw CfdgError(CFDG_ERR_CONFIG, "unknown limiter");
-        if (num->gradient < CFDG_GREEN_GAUSS_FACE || num->gradient > CFDG_LEAST_SQUARES_NODE)
-            throw CfdgError(CFDG_ERR_CONFIG, "unknown gradient");
-        if (!(num->gamma > 1.0) || !(num->gas_constant > 0.0) || !(num->cfl > 0.0))
-            throw CfdgError(CFDG_ERR_CONFIG, "gamma > 1, gas_constant > 0 and cfl > 0 required");
-        if (num->flow_model == CFDG_VISCOUS_FLOW && (!(num->prandtl > 0.0) || !(num->prandtl_turb > 0.0)))
-            throw CfdgError(CFDG_ERR_CONFIG, "viscous flow needs prandtl > 0 and prandtl_turb > 0");
-
-        int ndev = 0;
-        hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-        if (ndev < 1) throw CfdgError(CFDG_ERR_HIP, "no HIP device");
-        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_dots), 8 * sizeof(double)), "hipHostMalloc");
-
-        c->num = *num;
-        c->nc = m.n_cells;
-        c->nf = m.n_faces;
-        c->ninner = m.n_inner_faces;
-        c->nbf = m.n_faces - m.n_inner_faces;
-        c->ntot = c->nc + c->nbf;
-        c->npatch = n_bcs;
-        c->nargs.gas = cfdg::Gas{num->gamma, num->gas_constant, num->prandtl, num->prandtl_turb};
-        c->nargs.viscous = num->flow_model == CFDG_VISCOUS_FLOW;
-        c->nargs.muscl = num->reconstruction == CFDG_MUSCL;
-        c->nargs.limiter = num->limiter;
-        c->nargs.gradient = num->gradient;
-        c->nargs.k3 = num->venkat_k * num->venkat_k * num->venkat_k;
-        for (int32_t p = 0; p < n_bcs; ++p) c->bcs.bc[p] = resolve_bc(bcs[p], num->gamma, num->gas_constant);
-
-        hipStream_t s = c->stream;
-        c->owner.upload(m.face_owner, s);
-        c->neigh.upload(m.face_neigh, s);
-        c->fnx.upload(m.face_nx, s);
-        c->fny.upload(m.face_ny, s);
-        c->fnz.upload(m.face_nz, s);
-        c->farea.upload(m.face_area, s);
-        c->fcx.upload(m.face_cx, s);
-        c->fcy.upload(m.face_cy, s);
-        c->fcz.upload(m.face_cz, s);
-        c->ccx.upload(m.cell_cx, s);
-        c->ccy.upload(m.cell_cy, s);
-        c->ccz.upload(m.cell_cz, s);
-        c->vol.upload(m.cell_vol, s);
-        {
-            std::vector<double> dx, dy, dz, di;
-            cfdg::face_distances(m, dx, dy, dz, di);
-            c->dvx.upload(dx, s);
-            c->dvy.upload(dy, s);
-            c->dvz.upload(dz, s);
-            c->dinv.upload(di, s);
-            c->sync();
-        }
-        {
-            const cfdg::Csr cf = cfdg::cell_faces(m);
-            c->cf_off.upload(cf.off, s);
-            c->cf.upload(cf.idx, s);
-            c->ccf_host = cfdg::cell_cells_by_face(m);
-            c->ccf_off.upload(c->ccf_host.off, s);
-            c->ccf.upload(c->ccf_host.idx, s);
-            std::vector<int> bfb(static_cast<std::size_t>(c->nbf));
-            for (int i = 0; i < c->nbf; ++i) bfb[static_cast<std::size_t>(i)] = m.face_patch[static_cast<std::size_t>(c->ninner + i)];
-            c->bface_bc.upload(bfb, s);
-            c->sync();
-        }
-        if (num->gradient == CFDG_LEAST_SQUARES_NODE) {
-            const cfdg::Csr st = cfdg::cell_cells_by_node(m);
-            const std::vector<double> w = cfdg::lsq_coefficients(m, st);
-            c->ccn_off.upload(st.off, s);
-            c->ccn.upload(st.idx, s);
-            c->lsq.upload(w, s);
-            c->n_edges = static_cast<std::int64_t>(st.idx.size());
-            c->sync();
-        }
-        {
-            const cfdg::BsrPattern bp = cfdg::bsr_pattern(c->ccf_host, c->nc);
-            c->h_row_ptr = bp.row_ptr;
-            c->h_cols = bp.cols;
-            c->h_diag = bp.diag;
-            c->row_ptr.upload(bp.row_ptr, s);
-            c->cols.upload(bp.cols, s);
-            c->diag.upload(bp.diag, s);
-            std::int32_t ncol = 0;
-            const auto col = cfdg::greedy_colours(c->ccf_host, c->nc, ncol);
-            const auto gf = cfdg::groups_from_labels(col, ncol, false);
-            const auto gb = cfdg::groups_from_labels(col, ncol, true);
-            c->h_gofs[0] = gf.off;
-            c->h_gofs[1] = gb.off;
-            c->grows[0].upload(gf.rows, s);
-            c->grows[1].upload(gb.rows, s);
-            c->sync();
-        }
-        const std::size_t nt = static_cast<std::size_t>(c->ntot), ncs = static_cast<std::size_t>(c->nc);
-        c->U.alloc(5 * nt);
-        c->q.alloc(5 * nt);
-        c->grad.alloc(15 * nt);
-        c->phi.alloc(5 * nt);
-        c->res.alloc(5 * nt);
-        c->lam.alloc(ncs);
-        c->dt.alloc(ncs);
-        c->alpha.alloc(ncs);
-        c->U.zero(s);
-        c->q.zero(s);
-        c->grad.zero(s);
-        c->phi.zero(s);
-        c->res.zero(s);
-        c->lam.zero(s);
-        c->values.alloc(c->h_cols.size() * 25);
-        c->values.zero(s);
-        c->lu.alloc(ncs * 25);
-        c->piv.alloc(ncs * 5);
-        c->singular.alloc(1);
-        for (DBuf<double>* v : {&c->b, &c->x, &c->r, &c->r0, &c->p, &c->v, &c->s, &c->t, &c->ph, &c->sh}) {
-            v->alloc(5 * ncs);
-            v->zero(s);
-        }
-        c->partial.alloc(static_cast<std::size_t>(cfdg::kReduceBlocks) * 5);
-        c->dots.alloc(8);
-        c->sync();
-    });
+    const int rc = guarded(c, [&] { setup_context(*c, mesh->m, num, bcs, n_bcs); });
     if (rc != CFDG_OK) {
         std::fprintf(stderr, "cfdg_create: %s\n", c->err.c_str());
         delete c;
@@ -858,10 +771,10 @@ int cfdg_sizes(const cfdg_ctx* ctx, int32_t* n_cells, int32_t* n_total, int64_t*
 int cfdg_set_state(cfdg_ctx* c, const double* U, int64_t ld) {
     return guarded(c, [&] {
         if (!U || ld < c->nc) throw CfdgError(CFDG_ERR_ARG, "set_state: bad array");
+        std::vector<double> h(5 * z(c->nc));
         for (int v = 0; v < 5; ++v)
-            hip_check(hipMemcpyAsync(c->U.get() + static_cast<std::size_t>(v) * static_cast<std::size_t>(c->ntot), U + v * ld,
-                                     static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyHostToDevice, c->stream),
-                      "H2D");
+            for (int r = 0; r < c->nc; ++r) h[z(v) * z(c->nc) + z(r)] = U[v * ld + c->new2old[z(r)]];
+        c->h2d(c->U.get(), h.data(), h.size());
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
@@ -871,27 +784,26 @@ int cfdg_set_state(cfdg_ctx* c, const double* U, int64_t ld) {
 int cfdg_get_state(cfdg_ctx* c, double* U, int64_t ld) {
     return guarded(c, [&] {
         if (!U || ld < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_state: bad array");
-        for (int v = 0; v < 5; ++v)
-            hip_check(hipMemcpyAsync(U + v * ld, c->U.get() + static_cast<std::size_t>(v) * static_cast<std::size_t>(c->ntot),
-                                     static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                      "D2H");
+        std::vector<double> h(5 * z(c->nc));
+        c->d2h(h.data(), c->U.get(), h.size());
         c->sync();
+        for (int v = 0; v < 5; ++v)
+            for (int r = 0; r < c->nc; ++r) U[v * ld + c->new2old[z(r)]] = h[z(v) * z(c->nc) + z(r)];
     });
 }
 
 int cfdg_init_uniform(cfdg_ctx* c, double rho, const double vel[3], double p) {
     return guarded(c, [&] {
         if (!vel) throw CfdgError(CFDG_ERR_ARG, "init_uniform: null velocity");
-        // primitives_rhop_to_conserved (state_conversions.hpp:95-111), all n_total entries
+        // primitives_rhop_to_conserved (state_conversions.hpp:95-111)
         const double R = c->num.gas_constant, g = c->num.gamma;
         const double T = p / (rho * R);
         const double e = (R / (g - 1.0)) * T;
         const double ke = 0.5 * rho * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
         const double Uv[5] = {rho, rho * vel[0], rho * vel[1], rho * vel[2], rho * e + ke};
-        std::vector<double> h(static_cast<std::size_t>(c->ntot) * 5);
-        for (int v = 0; v < 5; ++v)
-            std::fill_n(h.begin() + static_cast<std::ptrdiff_t>(v) * c->ntot, c->ntot, Uv[v]);
-        hip_check(hipMemcpyAsync(c->U.get(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "H2D");
+        std::vector<double> h(z(c->nc) * 5);
+        for (int v = 0; v < 5; ++v) std::fill_n(h.begin() + static_cast<std::ptrdiff_t>(v) * c->nc, c->nc, Uv[v]);
+        c->h2d(c->U.get(), h.data(), h.size());
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
@@ -908,48 +820,43 @@ int cfdg_eval_residual(cfdg_ctx* c) {
 int cfdg_get_primitives(cfdg_ctx* c, double* q, int64_t ld) {
     return guarded(c, [&] {
         if (!q || ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_primitives: bad array");
-        for (int v = 0; v < 5; ++v)
-            hip_check(hipMemcpyAsync(q + v * ld, c->q.get() + static_cast<std::size_t>(v) * static_cast<std::size_t>(c->ntot),
-                                     static_cast<std::size_t>(c->ntot) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                      "D2H");
-        c->sync();
+        const auto h = c->download_records();
+        for (int r = 0; r < c->ntot; ++r) {
+            const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
+            for (int v = 0; v < 5; ++v) q[v * ld + o] = h[z(r) * cfdg::kRec + z(cfdg::kRQ + v)];
+        }
     });
 }
 
 int cfdg_get_gradients(cfdg_ctx* c, double* grad, int64_t ld, double* phi, int64_t ld_phi) {
     return guarded(c, [&] {
-        if (grad) {
-            if (ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld < n_total");
-            for (int k = 0; k < 15; ++k)
-                hip_check(hipMemcpyAsync(grad + k * ld, c->grad.get() + static_cast<std::size_t>(k) * static_cast<std::size_t>(c->ntot),
-                                         static_cast<std::size_t>(c->ntot) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                          "D2H");
+        if (grad && ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld < n_total");
+        if (phi && ld_phi < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld_phi < n_cells");
+        const auto h = c->download_records();
+        for (int r = 0; r < c->ntot; ++r) {
+            const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
+            const double* p = h.data() + z(r) * cfdg::kRec;
+            if (grad)
+                for (int k = 0; k < 15; ++k) grad[k * ld + o] = p[cfdg::kRG + k];
+            if (phi && r < c->nc)
+                for (int v = 0; v < 5; ++v) phi[v * ld_phi + o] = p[cfdg::kRPhi + v];
         }
-        if (phi) {
-            if (ld_phi < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld_phi < n_cells");
-            for (int v = 0; v < 5; ++v)
-                hip_check(hipMemcpyAsync(phi + v * ld_phi, c->phi.get() + static_cast<std::size_t>(v) * static_cast<std::size_t>(c->ntot),
-                                         static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                          "D2H");
-        }
-        c->sync();
     });
 }
 
 int cfdg_get_residual(cfdg_ctx* c, double* R, int64_t ld, double* lam) {
     return guarded(c, [&] {
-        if (R) {
-            if (ld < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_residual: ld < n_cells");
-            for (int v = 0; v < 5; ++v)
-                hip_check(hipMemcpyAsync(R + v * ld, c->res.get() + static_cast<std::size_t>(v) * static_cast<std::size_t>(c->ntot),
-                                         static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                          "D2H");
-        }
-        if (lam)
-            hip_check(hipMemcpyAsync(lam, c->lam.get(), static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost,
-                                     c->stream),
-                      "D2H");
+        if (R && ld < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_residual: ld < n_cells");
+        std::vector<double> h(5 * z(c->nc)), l(z(c->nc));
+        c->d2h(h.data(), c->res.get(), h.size());
+        c->d2h(l.data(), c->lam.get(), l.size());
         c->sync();
+        for (int r = 0; r < c->nc; ++r) {
+            const long long o = c->new2old[z(r)];
+            if (R)
+                for (int v = 0; v < 5; ++v) R[v * ld + o] = h[z(v) * z(c->nc) + z(r)];
+            if (lam) lam[o] = l[z(r)];
+        }
     });
 }
 
@@ -964,14 +871,14 @@ int cfdg_compute_dt(cfdg_ctx* c) { return guarded(c, [&] { compute_dt(*c); }); }
 
 int cfdg_get_dt(cfdg_ctx* c, double* dt, double* alpha) {
     return guarded(c, [&] {
-        if (dt)
-            hip_check(hipMemcpyAsync(dt, c->dt.get(), static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                      "D2H");
-        if (alpha)
-            hip_check(hipMemcpyAsync(alpha, c->alpha.get(), static_cast<std::size_t>(c->nc) * sizeof(double), hipMemcpyDeviceToHost,
-                                     c->stream),
-                      "D2H");
+        std::vector<double> a(z(c->nc)), b(z(c->nc));
+        c->d2h(a.data(), c->dt.get(), a.size());
+        c->d2h(b.data(), c->alpha.get(), b.size());
         c->sync();
+        for (int r = 0; r < c->nc; ++r) {
+            if (dt) dt[c->new2old[z(r)]] = a[z(r)];
+            if (alpha) alpha[c->new2old[z(r)]] = b[z(r)];
+        }
     });
 }
 
@@ -988,17 +895,21 @@ int cfdg_get_bsr_pattern(cfdg_ctx* c, int32_t* row_ptr, int32_t* cols, int32_t* 
 int cfdg_get_bsr_values(cfdg_ctx* c, double* values) {
     return guarded(c, [&] {
         if (!values) throw CfdgError(CFDG_ERR_ARG, "null output");
-        hip_check(hipMemcpyAsync(values, c->values.get(), c->values.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-                  "D2H");
+        std::vector<double> h(c->values.size());
+        c->d2h(h.data(), c->values.get(), h.size());
         c->sync();
+        for (std::size_t k = 0; k < c->blk_dev.size(); ++k)
+            std::memcpy(values + k * 25, h.data() + z(c->blk_dev[k]) * 25, 25 * sizeof(double));
     });
 }
 
 int cfdg_set_bsr_values(cfdg_ctx* c, const double* values) {
     return guarded(c, [&] {
         if (!values) throw CfdgError(CFDG_ERR_ARG, "null input");
-        hip_check(hipMemcpyAsync(c->values.get(), values, c->values.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
-                  "H2D");
+        std::vector<double> h(c->values.size());
+        for (std::size_t k = 0; k < c->blk_dev.size(); ++k)
+            std::memcpy(h.data() + z(c->blk_dev[k]) * 25, values + k * 25, 25 * sizeof(double));
+        c->h2d(c->values.get(), h.data(), h.size());
         c->sync();
     });
 }
@@ -1011,33 +922,39 @@ int cfdg_linear_solve(cfdg_ctx* c, const cfdg_linparams* lp, const double* b, co
     });
 }
 
-int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* z) {
+int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* zout) {
     return guarded(c, [&] {
-        if (!r || !z) throw CfdgError(CFDG_ERR_ARG, "null vector");
+        if (!r || !zout) throw CfdgError(CFDG_ERR_ARG, "null vector");
         if (sgs_order != CFDG_SGS_MULTICOLOUR && sgs_order != CFDG_SGS_SEQUENTIAL)
             throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
-        const std::size_t bytes = 5ULL * static_cast<std::size_t>(c->nc) * sizeof(double);
+        const std::size_t n5 = 5ULL * z(c->nc);
         Precond P;
         P.kind = CFDG_PRECOND_SGS;
         P.order = sgs_order;
         precond_setup(*c, P);
-        hip_check(hipMemcpyAsync(c->s.get(), r, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+        const auto d = c->vec_to_device(r);
+        c->h2d(c->s.get(), d.data(), n5);
         precond_apply(*c, P, c->s.get(), c->sh.get());
-        hip_check(hipMemcpyAsync(z, c->sh.get(), bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+        std::vector<double> h(n5);
+        c->d2h(h.data(), c->sh.get(), n5);
         c->check_launch();
         c->sync();
+        c->vec_to_mesh(h, zout);
     });
 }
 
 int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
     return guarded(c, [&] {
         if (!x || !y) throw CfdgError(CFDG_ERR_ARG, "null vector");
-        const std::size_t bytes = 5ULL * static_cast<std::size_t>(c->nc) * sizeof(double);
-        hip_check(hipMemcpyAsync(c->s.get(), x, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+        const std::size_t n5 = 5ULL * z(c->nc);
+        const auto d = c->vec_to_device(x);
+        c->h2d(c->s.get(), d.data(), n5);
         cfdg::launch_spmv(c->bargs(), c->values.get(), c->s.get(), c->t.get(), c->stream);
-        hip_check(hipMemcpyAsync(y, c->t.get(), bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+        std::vector<double> h(n5);
+        c->d2h(h.data(), c->t.get(), n5);
         c->check_launch();
         c->sync();
+        c->vec_to_mesh(h, y);
     });
 }
 
@@ -1051,7 +968,7 @@ int cfdg_implicit_step(cfdg_ctx* c, const cfdg_linparams* lp, cfdg_step_result* 
         linear_solve(*c, *lp, nullptr, nullptr, nullptr, &lr);
         double l2[5];
         residual_norms(*c, l2);
-        cfdg::launch_state_update(c->nc, c->ntot, c->x.get(), c->U.get(), c->stream);
+        cfdg::launch_state_update(c->nc, c->nc, c->x.get(), c->U.get(), c->stream);
         c->check_launch();
         c->sync();
         c->have_residual = false;

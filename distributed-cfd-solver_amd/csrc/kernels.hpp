@@ -1,5 +1,10 @@
-// Kernel argument blocks and launch wrappers shared by the host context code
-// and the HIP translation units.  All device pointers; passed by value.
+// Kernel argument blocks, device data layouts and launch wrappers shared by the
+// host context code and the HIP translation units.  All pointers are device
+// pointers; blocks are passed by value.
+//
+// Device cell numbering: cells are renumbered into compact tiles (setup.hpp,
+// tile_cells); faces keep their mesh index, so every per-cell sum over faces
+// runs in ascending face index exactly as the reference's face sweeps do.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,48 +16,50 @@
 namespace cfdg {
 
 constexpr int kMaxPatches = 64;
+constexpr int kTileCells = 256;       // cells per tile (= threads per flux workgroup)
+constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux kernel
+constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 
-// Mesh in device memory: face arrays in the 3-zone order, cell-centric CSR lists.
+// Cell record, 32 doubles (256 B, two 128-B lines), one per cell and per
+// boundary ghost: line 0 = what neighbour gathers need (primitives, centroid,
+// limiter, volume), line 1 = the 15 gradient components (3 v + d).
+constexpr int kRec = 32;
+constexpr int kRQ = 0;     // p, u, v, w, T
+constexpr int kRC = 5;     // centroid x, y, z
+constexpr int kRPhi = 8;   // limiter of the 5 primitives
+constexpr int kRVol = 13;  // volume
+constexpr int kRG = 16;    // gradients, (3 v + d)
+
+// Face record, 16 doubles (128 B, one line): unit normal, area, centroid,
+// owner->neighbour centroid vector (mirrored 2 (x_f - x_c) at boundary faces),
+// its inverse length, then owner and neighbour as two int32 in one slot.
+constexpr int kFRec = 16;
+constexpr int kFN = 0, kFA = 3, kFC = 4, kFD = 7, kFDinv = 10, kFCells = 11;
+
 struct MeshArgs {
-    int nc;       // cells
-    int nf;       // faces
-    int ninner;   // faces with a neighbour cell
-    int ld;       // plane stride of the field arrays (n_total = nc + boundary faces)
-    const int* owner;
-    const int* neigh;
-    const double* fnx;
-    const double* fny;
-    const double* fnz;
-    const double* farea;
-    const double* fcx;
-    const double* fcy;
-    const double* fcz;
-    const double* dvx;   // owner -> neighbour (or mirrored ghost) centroid vector
-    const double* dvy;
-    const double* dvz;
-    const double* dinv;  // 1 / |d|
-    const double* ccx;
-    const double* ccy;
-    const double* ccz;
-    const double* vol;
-    const int* cf_off;   // cell -> faces, ascending face index
+    int nc;        // cells
+    int nf;        // faces
+    int ninner;    // faces with a neighbour cell
+    int ntiles;
+    const double* frec;
+    const int* cf_off;        // device cell -> faces, ascending face index
     const int* cf;
-    const int* ccf_off;  // cell -> face neighbours, ascending cell index
+    const int* cf_slot;       // position of the face in the tile's face list (negative: neighbour side)
+    const int* ccf_off;       // device cell -> face neighbours (device ids, ascending mesh index)
     const int* ccf;
-    const int* ccn_off;  // cell -> vertex neighbours (LSQ stencil), ascending
+    const int* ccn_off;       // device cell -> vertex neighbours (LSQ stencil), ascending mesh index
     const int* ccn;
-    const double* lsq;   // LSQ coefficients, three planes of n_edges
-    std::int64_t n_edges;
-    const int* bface_bc; // boundary face -> patch id
+    const int* bface_bc;      // boundary face -> patch id
+    const int* tile_off;      // tiles + 1
+    const int* tile_face_off; // tiles + 1
+    const int* tile_faces;
 };
 
 struct FieldArgs {
-    const double* U;  // conservative, 5 planes
-    double* q;        // primitive (p, u, v, w, T), 5 planes incl. ghosts
-    double* grad;     // 15 planes: (3 v + d)
-    double* phi;      // 5 planes (cells only)
-    double* res;      // 5 planes
-    double* lam;      // spectral radius per cell
+    const double* U;  // conservative, 5 planes of nc (device order)
+    double* crec;     // cell records, (nc + boundary faces) x kRec
+    double* res;      // 5 planes of nc
+    double* lam;      // nc
 };
 
 struct NumArgs {
@@ -75,6 +82,15 @@ struct BsrArgs {
     const int* diag;
 };
 
+// Bijective block -> work-item map that gives each XCD one contiguous range of
+// the work (workgroups are dealt to the 8 XCDs round-robin), so cells that
+// share neighbours are processed by the same L2.
+__host__ __device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % kNumXcd, i = b / kNumXcd;
+    const int base = nb / kNumXcd, rem = nb % kNumXcd;
+    return x * base + (x < rem ? x : rem) + i;
+}
+
 // --- residual stages (kernels_residual.hip) ---------------------------------
 void launch_primitives(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
 void launch_ghosts(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs, hipStream_t s);
@@ -82,7 +98,8 @@ void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,
                             hipStream_t s);
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
-void launch_dt(const MeshArgs& m, const double* lam, double cfl, double* dt, double* alpha, hipStream_t s);
+void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
+               hipStream_t s);
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
                      const double* alpha, double* values, hipStream_t s);
 

@@ -1,11 +1,22 @@
 // Residual, time-step and Jacobian kernels (gfx950).
 //
-// Design: every kernel is cell-centric (one thread per cell, or per boundary
-// face for the ghost stages).  Face contributions are gathered over the
-// cell's face list in ascending face index -- the order in which the
-// reference's face sweeps scatter into a cell -- so each cell sum is formed
-// in the same order, without atomics, and is reproducible run to run.  The
-// price is that an interior face's flux is evaluated by both adjacent cells.
+// Data: cell records (256 B) and face records (128 B), kernels.hpp.  Cells are
+// in the device (tile) order; faces keep their mesh index.
+//
+//   k_primitives        U -> (p,u,v,w,T) into the cell records
+//   k_ghosts            boundary ghost states, one thread per boundary face
+//   k_gradient_limiter  cell gradient (node-stencil least squares with the
+//                       3x3 normal equations rebuilt per cell from neighbour
+//                       centroids, or Green-Gauss) + MUSCL limiter, one thread
+//                       per cell
+//   k_ghost_gradients   boundary ghost gradients
+//   k_flux              tiled flux residual: a workgroup owns a tile of cells,
+//                       evaluates each face touching the tile once into LDS,
+//                       then every cell sums its faces in ascending face index
+//   k_dt, k_jacobian    local time step; Picard Jacobian, one thread per block row
+//
+// No floating-point atomics: every output element has a single writer and
+// every sum has a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -16,39 +27,42 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ const double* plane(const double* a, int ld, int v) { return a + static_cast<long long>(v) * ld; }
-__device__ __forceinline__ double* plane(double* a, int ld, int v) { return a + static_cast<long long>(v) * ld; }
+__device__ __forceinline__ int face_owner(const double* fr) { return reinterpret_cast<const int*>(fr + kFCells)[0]; }
+__device__ __forceinline__ int face_neigh(const double* fr) { return reinterpret_cast<const int*>(fr + kFCells)[1]; }
+__device__ __forceinline__ const double* frec(const MeshArgs& m, int f) { return m.frec + static_cast<long long>(f) * kFRec; }
+__device__ __forceinline__ double* crec(const FieldArgs& f, int c) { return f.crec + static_cast<long long>(c) * kRec; }
 
-// ---------------------------------------------------------------------------
 // U -> q for the cells (solver.hpp:185-188)
 __global__ void __launch_bounds__(kBlock) k_primitives(MeshArgs m, FieldArgs f, NumArgs nm) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
     double U[5], q[5];
-    for (int v = 0; v < 5; ++v) U[v] = plane(f.U, m.ld, v)[c];
+    for (int v = 0; v < 5; ++v) U[v] = f.U[static_cast<long long>(v) * m.nc + c];
     cons_to_prim(nm.gas, U, q);
-    for (int v = 0; v < 5; ++v) plane(f.q, m.ld, v)[c] = q[v];
+    double* r = crec(f, c);
+    for (int v = 0; v < 5; ++v) r[kRQ + v] = q[v];
 }
 
-__device__ __forceinline__ double rcfn_inverse(const MeshArgs& m, int f, int in) {
-    // boundary_traversal.hpp:98-108
-    const double rx = m.fcx[f] - m.ccx[in], ry = m.fcy[f] - m.ccy[in], rz = m.fcz[f] - m.ccz[in];
-    const double r = rx * m.fnx[f] + ry * m.fny[f] + rz * m.fnz[f];
+// 1 / max((x_f - x_c) . n, 1e-14)   (boundary_traversal.hpp:98-108)
+__device__ __forceinline__ double rcfn_inverse(const double* fr, const double* cr) {
+    const double rx = fr[kFC] - cr[kRC], ry = fr[kFC + 1] - cr[kRC + 1], rz = fr[kFC + 2] - cr[kRC + 2];
+    const double r = rx * fr[kFN] + ry * fr[kFN + 1] + rz * fr[kFN + 2];
     return 1.0 / dmax(r, 1.0e-14);
 }
 
-// Ghost primitive states, one thread per boundary face (bc/physical/*.hpp value kernels)
+// Ghost primitive states (bc/physical/*.hpp value kernels)
 __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumArgs nm, BcTable bt) {
     const int bf = blockIdx.x * blockDim.x + threadIdx.x;
     const int nbf = m.nf - m.ninner;
     if (bf >= nbf) return;
     const int fc = m.ninner + bf;
-    const int in = m.owner[fc];
-    const int gh = m.nc + bf;
+    const double* fr = frec(m, fc);
+    const double* ri = crec(f, face_owner(fr));
+    double* rg = crec(f, m.nc + bf);
     const BcDev bp = bt.bc[m.bface_bc[bf]];
     double qi[5], qg[5];
-    for (int v = 0; v < 5; ++v) qi[v] = plane(f.q, m.ld, v)[in];
-    const double nx = m.fnx[fc], ny = m.fny[fc], nz = m.fnz[fc];
+    for (int v = 0; v < 5; ++v) qi[v] = ri[kRQ + v];
+    const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2];
     switch (bp.type) {
         case kSupIn: {
             const double qf[5] = {bp.p, bp.u, bp.v, bp.w, bp.T};
@@ -81,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
             break;
         }
         case kNoSlip:
-        case kSubIn: {
+        case kSubIn:
             qg[0] = qi[0];
             qg[1] = 2.0 * bp.u - qi[1];
             qg[2] = 2.0 * bp.v - qi[2];
@@ -89,9 +103,8 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
             qg[4] = 2.0 * bp.T - qi[4];
             qg[4] = dmax(qg[4], 1.0);
             break;
-        }
         case kNoSlipHF: {
-            const double rinv = rcfn_inverse(m, fc, in);
+            const double rinv = rcfn_inverse(fr, ri);
             qg[0] = qi[0];
             qg[1] = 2.0 * bp.u - qi[1];
             qg[2] = 2.0 * bp.v - qi[2];
@@ -101,57 +114,101 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
             qg[4] = dmax(qg[4], 1.0);
             break;
         }
-        case kSubOut: {
+        case kSubOut:
             qg[0] = 2.0 * bp.p - qi[0];
             qg[0] = dmax(qg[0], 1.0);
             for (int v = 1; v < 5; ++v) qg[v] = qi[v];
             break;
-        }
-        default:  // kSupOut: extrapolation
+        default:  // supersonic outlet: extrapolation
             for (int v = 0; v < 5; ++v) qg[v] = qi[v];
             break;
     }
-    for (int v = 0; v < 5; ++v) plane(f.q, m.ld, v)[gh] = qg[v];
+    for (int v = 0; v < 5; ++v) rg[kRQ + v] = qg[v];
 }
 
-// Cell gradients (Green-Gauss face/cell or node-stencil least squares) fused
-// with the MUSCL limiter of the same cell (muscl.hpp:26-104): the limiter only
-// needs the cell's own gradient, which stays in registers.
+// Cell gradient + MUSCL limiter (gradient_operator.cpp:549-707, :197-278;
+// muscl.hpp:26-104), one thread per cell.
 __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldArgs f, NumArgs nm) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
+    double* rc = crec(f, c);
     double qc[5];
-    for (int v = 0; v < 5; ++v) qc[v] = plane(f.q, m.ld, v)[c];
+    for (int v = 0; v < 5; ++v) qc[v] = rc[kRQ + v];
+    const double x0 = rc[kRC], y0 = rc[kRC + 1], z0 = rc[kRC + 2];
     double g[5][3];
     for (int v = 0; v < 5; ++v) g[v][0] = g[v][1] = g[v][2] = 0.0;
 
     if (nm.gradient == 2) {
-        // gradient_operator.cpp:549-599
-        const double* wx = m.lsq;
-        const double* wy = m.lsq + m.n_edges;
-        const double* wz = m.lsq + 2 * m.n_edges;
-        for (int j = m.ccn_off[c]; j < m.ccn_off[c + 1]; ++j) {
-            const int nb = m.ccn[j];
-            const double cx = wx[j], cy = wy[j], cz = wz[j];
-            for (int v = 0; v < 5; ++v) {
-                const double d = plane(f.q, m.ld, v)[nb] - qc[v];
-                g[v][0] += cx * d;
-                g[v][1] += cy * d;
-                g[v][2] += cz * d;
+        // weighted least squares over the vertex-sharing stencil: the normal
+        // matrix A = sum w dr dr^T + 1e-14 I (w = 1/|dr|^2) is rebuilt from the
+        // neighbour centroids, inverted by the adjugate, and the coefficients
+        // C_j = A^-1 w_j dr_j are applied on the fly -- the same arithmetic as
+        // the reference's set-up-time coefficients.
+        const int jb = m.ccn_off[c], je = m.ccn_off[c + 1];
+        if (je > jb) {
+            double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
+            for (int j = jb; j < je; ++j) {
+                const double* rn = crec(f, m.ccn[j]);
+                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double w = 1.0 / dmax(r2, 1.0e-30);
+                A0 += w * dx * dx;
+                A1 += w * dx * dy;
+                A2 += w * dx * dz;
+                A4 += w * dy * dy;
+                A5 += w * dy * dz;
+                A8 += w * dz * dz;
+            }
+            const double A3 = A1, A6 = A2, A7 = A5;
+            A0 += 1.0e-14;
+            A4 += 1.0e-14;
+            A8 += 1.0e-14;
+            double inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const double det = A0 * (A4 * A8 - A5 * A7) - A1 * (A3 * A8 - A5 * A6) + A2 * (A3 * A7 - A4 * A6);
+            if (!(fabs(det) <= 1.0e-14)) {
+                const double id = 1.0 / det;
+                inv[0] = (A4 * A8 - A5 * A7) * id;
+                inv[1] = (A2 * A7 - A1 * A8) * id;
+                inv[2] = (A1 * A5 - A2 * A4) * id;
+                inv[3] = (A5 * A6 - A3 * A8) * id;
+                inv[4] = (A0 * A8 - A2 * A6) * id;
+                inv[5] = (A2 * A3 - A0 * A5) * id;
+                inv[6] = (A3 * A7 - A4 * A6) * id;
+                inv[7] = (A1 * A6 - A0 * A7) * id;
+                inv[8] = (A0 * A4 - A1 * A3) * id;
+            }
+            for (int j = jb; j < je; ++j) {
+                const double* rn = crec(f, m.ccn[j]);
+                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double w = 1.0 / dmax(r2, 1.0e-30);
+                const double rx = w * dx, ry = w * dy, rz = w * dz;
+                const double cx = inv[0] * rx + inv[1] * ry + inv[2] * rz;
+                const double cy = inv[3] * rx + inv[4] * ry + inv[5] * rz;
+                const double cz = inv[6] * rx + inv[7] * ry + inv[8] * rz;
+                for (int v = 0; v < 5; ++v) {
+                    const double d = rn[kRQ + v] - qc[v];
+                    g[v][0] += cx * d;
+                    g[v][1] += cy * d;
+                    g[v][2] += cz * d;
+                }
             }
         }
     } else {
-        // Green-Gauss: face scatter of gradient_operator.cpp:197-278 gathered per cell
-        // (the cell-based variant, :61-130, forms the same sums)
+        // Green-Gauss: the face scatter gathered per cell (both the face- and
+        // cell-based variants form these sums)
         for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
             const int fc = m.cf[e];
-            const int o = m.owner[fc];
-            const int nb = (fc < m.ninner) ? m.neigh[fc] : m.nc + (fc - m.ninner);
-            const double a05 = 0.5 * m.farea[fc];
-            const double hx = a05 * m.fnx[fc], hy = a05 * m.fny[fc], hz = a05 * m.fnz[fc];
+            const double* fr = frec(m, fc);
+            const int o = face_owner(fr);
+            const int nb = (fc < m.ninner) ? face_neigh(fr) : m.nc + (fc - m.ninner);
+            const double a05 = 0.5 * fr[kFA];
+            const double hx = a05 * fr[kFN], hy = a05 * fr[kFN + 1], hz = a05 * fr[kFN + 2];
             const bool own = (o == c);
+            const double* ro = crec(f, o);
+            const double* rn = crec(f, nb);
             for (int v = 0; v < 5; ++v) {
-                const double s = plane(f.q, m.ld, v)[o] + plane(f.q, m.ld, v)[nb];
+                const double s = ro[kRQ + v] + rn[kRQ + v];
                 const double fx = hx * s, fy = hy * s, fz = hz * s;
                 if (own) {
                     g[v][0] += fx;
@@ -164,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
                 }
             }
         }
-        const double iv = 1.0 / m.vol[c];
+        const double iv = 1.0 / rc[kRVol];
         for (int v = 0; v < 5; ++v) {
             g[v][0] *= iv;
             g[v][1] *= iv;
@@ -172,32 +229,31 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
         }
     }
     for (int v = 0; v < 5; ++v)
-        for (int d = 0; d < 3; ++d) plane(f.grad, m.ld, 3 * v + d)[c] = g[v][d];
+        for (int d = 0; d < 3; ++d) rc[kRG + 3 * v + d] = g[v][d];
 
     if (!nm.muscl) return;
     double qmax[5], qmin[5];
     for (int v = 0; v < 5; ++v) qmax[v] = qmin[v] = qc[v];
     for (int j = m.ccf_off[c]; j < m.ccf_off[c + 1]; ++j) {
-        const int nb = m.ccf[j];
+        const double* rn = crec(f, m.ccf[j]);
         for (int v = 0; v < 5; ++v) {
-            const double qn = plane(f.q, m.ld, v)[nb];
+            const double qn = rn[kRQ + v];
             qmax[v] = dmax(qmax[v], qn);
             qmin[v] = dmin(qmin[v], qn);
         }
     }
     double ph[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
-    const double eps2 = nm.k3 * m.vol[c];
-    const double cx = m.ccx[c], cy = m.ccy[c], cz = m.ccz[c];
+    const double eps2 = nm.k3 * rc[kRVol];
     for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const int fc = m.cf[e];
-        const double ex = m.fcx[fc] - cx, ey = m.fcy[fc] - cy, ez = m.fcz[fc] - cz;
+        const double* fr = frec(m, m.cf[e]);
+        const double ex = fr[kFC] - x0, ey = fr[kFC + 1] - y0, ez = fr[kFC + 2] - z0;
         for (int v = 0; v < 5; ++v) {
             const double df = g[v][0] * ex + g[v][1] * ey + g[v][2] * ez;
             const double dnb = (df > 0.0) ? (qmax[v] - qc[v]) : (qmin[v] - qc[v]);
             ph[v] = dmin(ph[v], limiter_phi(nm.limiter, dnb, df, eps2));
         }
     }
-    for (int v = 0; v < 5; ++v) plane(f.phi, m.ld, v)[c] = ph[v];
+    for (int v = 0; v < 5; ++v) rc[kRPhi + v] = ph[v];
 }
 
 __device__ __forceinline__ void grad_dirichlet(double* g, const double* gi, double qin, double qfix, double nx,
@@ -229,22 +285,22 @@ __device__ __forceinline__ void grad_slip(double* g, const double* gi, double qi
     g[2] = gi[2] + fac * nz;
 }
 
-// Ghost gradients, one thread per boundary face (bc/physical/*.hpp gradient kernels,
-// numerics/boundary/boundary_gradients.hpp)
+// Ghost gradients (bc/physical/*.hpp gradient kernels, boundary_gradients.hpp)
 __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArgs f, NumArgs nm, BcTable bt) {
     const int bf = blockIdx.x * blockDim.x + threadIdx.x;
     const int nbf = m.nf - m.ninner;
     if (bf >= nbf) return;
     const int fc = m.ninner + bf;
-    const int in = m.owner[fc];
-    const int gh = m.nc + bf;
+    const double* fr = frec(m, fc);
+    const double* ri = crec(f, face_owner(fr));
+    double* rg = crec(f, m.nc + bf);
     const BcDev bp = bt.bc[m.bface_bc[bf]];
-    const double nx = m.fnx[fc], ny = m.fny[fc], nz = m.fnz[fc];
-    const double rinv = rcfn_inverse(m, fc, in);
+    const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2];
+    const double rinv = rcfn_inverse(fr, ri);
     double gi[5][3], go[5][3], qi[5];
     for (int v = 0; v < 5; ++v) {
-        qi[v] = plane(f.q, m.ld, v)[in];
-        for (int d = 0; d < 3; ++d) go[v][d] = gi[v][d] = plane(f.grad, m.ld, 3 * v + d)[in];
+        qi[v] = ri[kRQ + v];
+        for (int d = 0; d < 3; ++d) go[v][d] = gi[v][d] = ri[kRG + 3 * v + d];
     }
     switch (bp.type) {
         case kSupIn: {
@@ -255,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
         case kSlip:
         case kSym:
             grad_neumann(go[0], gi[0], 0.0, nx, ny, nz);
-            for (int v = 1; v < 4; ++v) grad_slip(go[v], gi[v], qi[v], plane(f.q, m.ld, v)[gh], nx, ny, nz, rinv);
+            for (int v = 1; v < 4; ++v) grad_slip(go[v], gi[v], qi[v], rg[kRQ + v], nx, ny, nz, rinv);
             grad_neumann(go[4], gi[4], 0.0, nx, ny, nz);
             break;
         case kFar: {
@@ -281,90 +337,112 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
         case kSubOut:
             grad_dirichlet(go[0], gi[0], qi[0], bp.p, nx, ny, nz, rinv);
             break;
-        default:  // kSupOut: extrapolation
+        default:  // supersonic outlet: extrapolation
             break;
     }
     for (int v = 0; v < 5; ++v)
-        for (int d = 0; d < 3; ++d) plane(f.grad, m.ld, 3 * v + d)[gh] = go[v][d];
+        for (int d = 0; d < 3; ++d) rg[kRG + 3 * v + d] = go[v][d];
 }
 
-// Flux residual: R_c = sum over the cell's faces of +-(F_inv + F_visc) A and
-// lambda_c = sum of A s_max + lambda_visc (residual_kernel.hpp:175-306), with
-// MUSCL face states (muscl.hpp:108-157).
-__global__ void __launch_bounds__(kBlock) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nc) return;
+// Inviscid + viscous flux of one face times its area, and its spectral-radius
+// contribution (residual_kernel.hpp:183-240 / 245-300 for one face, with the
+// MUSCL states of muscl.hpp:108-157).
+__device__ __forceinline__ void face_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int fc,
+                                          double F[5], double& lam_f) {
+    const double* fr = frec(m, fc);
+    const bool inner = fc < m.ninner;
+    const int c0 = face_owner(fr);
+    const int c1 = inner ? face_neigh(fr) : m.nc + (fc - m.ninner);
+    const double* r0 = crec(f, c0);
+    const double* r1 = crec(f, c1);
+    const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
+    double qL[5], qR[5];
+    if (inner && nm.muscl) {
+        const double fx = fr[kFC], fy = fr[kFC + 1], fz = fr[kFC + 2];
+        const double d0x = fx - r0[kRC], d0y = fy - r0[kRC + 1], d0z = fz - r0[kRC + 2];
+        const double d1x = fx - r1[kRC], d1y = fy - r1[kRC + 1], d1z = fz - r1[kRC + 2];
+        for (int v = 0; v < 5; ++v) {
+            const double dot0 = r0[kRG + 3 * v] * d0x + r0[kRG + 3 * v + 1] * d0y + r0[kRG + 3 * v + 2] * d0z;
+            qL[v] = r0[kRQ + v] + r0[kRPhi + v] * dot0;
+            const double dot1 = r1[kRG + 3 * v] * d1x + r1[kRG + 3 * v + 1] * d1y + r1[kRG + 3 * v + 2] * d1z;
+            qR[v] = r1[kRQ + v] + r1[kRPhi + v] * dot1;
+        }
+    } else {
+        for (int v = 0; v < 5; ++v) {
+            qL[v] = r0[kRQ + v];
+            qR[v] = r1[kRQ + v];
+        }
+    }
+    double UL[5], UR[5];
+    prim_to_cons(nm.gas, qL, UL);
+    prim_to_cons(nm.gas, qR, UR);
+    double smax = 0.0;
+    hllc_flux(nm.gas, UL, UR, nx, ny, nz, A, F, smax);
+    lam_f = A * smax;
+    if (nm.viscous) {
+        double qa[5], qb[5], ga[12], gb[12];
+        for (int v = 0; v < 5; ++v) {
+            qa[v] = r0[kRQ + v];
+            qb[v] = r1[kRQ + v];
+        }
+        for (int k = 0; k < 12; ++k) {
+            ga[k] = r0[kRG + 3 + k];
+            gb[k] = r1[kRG + 3 + k];
+        }
+        double Fv[5];
+        double lv = 0.0;
+        viscous_flux(nm.gas, fr[kFDinv], fr[kFD], fr[kFD + 1], fr[kFD + 2], qa, qb, ga, gb, 0.0, 0.0, nx, ny, nz, A, Fv,
+                     lv);
+        for (int v = 1; v < 5; ++v) F[v] += Fv[v];
+        lam_f += lv;
+    }
+}
+
+// Tiled flux residual: each face touching the tile is evaluated once (face-
+// parallel) into LDS, then each cell adds its faces' fluxes in ascending face
+// index, with + for the owner and - for the neighbour (residual_kernel.hpp:
+// 175-306 formed as a gather; only faces on a tile boundary are evaluated by
+// both adjacent tiles).
+__global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
+    __shared__ double sF[6][kTileFaces];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
+    for (int i = threadIdx.x; i < nfl; i += kTileCells) {
+        double F[5], lam_f;
+        face_flux(m, f, nm, m.tile_faces[fb + i], F, lam_f);
+        for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
+        sF[5][i] = lam_f;
+    }
+    __syncthreads();
+    const int c = m.tile_off[t] + threadIdx.x;
+    if (c >= m.tile_off[t + 1]) return;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lam = 0.0;
     for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const int fc = m.cf[e];
-        const bool inner = fc < m.ninner;
-        const int c0 = m.owner[fc];
-        const int c1 = inner ? m.neigh[fc] : m.nc + (fc - m.ninner);
-        const double nx = m.fnx[fc], ny = m.fny[fc], nz = m.fnz[fc], A = m.farea[fc];
-        double qa[5], qb[5];
-        for (int v = 0; v < 5; ++v) {
-            qa[v] = plane(f.q, m.ld, v)[c0];
-            qb[v] = plane(f.q, m.ld, v)[c1];
-        }
-        double qL[5], qR[5];
-        if (inner && nm.muscl) {
-            const double fx = m.fcx[fc], fy = m.fcy[fc], fz = m.fcz[fc];
-            const double d0x = fx - m.ccx[c0], d0y = fy - m.ccy[c0], d0z = fz - m.ccz[c0];
-            const double d1x = fx - m.ccx[c1], d1y = fy - m.ccy[c1], d1z = fz - m.ccz[c1];
-            for (int v = 0; v < 5; ++v) {
-                const double dot0 = plane(f.grad, m.ld, 3 * v)[c0] * d0x + plane(f.grad, m.ld, 3 * v + 1)[c0] * d0y +
-                                    plane(f.grad, m.ld, 3 * v + 2)[c0] * d0z;
-                qL[v] = qa[v] + plane(f.phi, m.ld, v)[c0] * dot0;
-                const double dot1 = plane(f.grad, m.ld, 3 * v)[c1] * d1x + plane(f.grad, m.ld, 3 * v + 1)[c1] * d1y +
-                                    plane(f.grad, m.ld, 3 * v + 2)[c1] * d1z;
-                qR[v] = qb[v] + plane(f.phi, m.ld, v)[c1] * dot1;
-            }
+        const int s = m.cf_slot[e];
+        if (s >= 0) {
+            for (int v = 0; v < 5; ++v) acc[v] += sF[v][s];
+            lam += sF[5][s];
         } else {
-            for (int v = 0; v < 5; ++v) {
-                qL[v] = qa[v];
-                qR[v] = qb[v];
-            }
+            const int k = -s - 1;
+            for (int v = 0; v < 5; ++v) acc[v] -= sF[v][k];
+            lam += sF[5][k];
         }
-        double UL[5], UR[5], F[5];
-        prim_to_cons(nm.gas, qL, UL);
-        prim_to_cons(nm.gas, qR, UR);
-        double smax = 0.0;
-        hllc_flux(nm.gas, UL, UR, nx, ny, nz, A, F, smax);
-        double lam_f = A * smax;
-        if (nm.viscous) {
-            double ga[12], gb[12];
-            for (int k = 0; k < 12; ++k) {
-                ga[k] = plane(f.grad, m.ld, 3 + k)[c0];
-                gb[k] = plane(f.grad, m.ld, 3 + k)[c1];
-            }
-            double Fv[5];
-            double lv = 0.0;
-            viscous_flux(nm.gas, m.dinv[fc], m.dvx[fc], m.dvy[fc], m.dvz[fc], qa, qb, ga, gb, 0.0, 0.0, nx, ny, nz, A,
-                         Fv, lv);
-            for (int v = 1; v < 5; ++v) F[v] += Fv[v];
-            lam_f += lv;
-        }
-        if (c0 == c) {
-            for (int v = 0; v < 5; ++v) acc[v] += F[v];
-        } else {
-            for (int v = 0; v < 5; ++v) acc[v] -= F[v];
-        }
-        lam += lam_f;
     }
-    for (int v = 0; v < 5; ++v) plane(f.res, m.ld, v)[c] = acc[v];
+    for (int v = 0; v < 5; ++v) f.res[static_cast<long long>(v) * m.nc + c] = acc[v];
     f.lam[c] = lam;
 }
 
 // Implicit local time step and diagonal shift (solver.hpp:256-287)
-__global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* vol, const double* lam, double cfl, double* dt,
+__global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cr, const double* lam, double cfl, double* dt,
                                                double* alpha) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
+    const double vol = cr[static_cast<long long>(c) * kRec + kRVol];
     const double inv_cfl = 1.0 / cfl;
     const double l = dmax(lam[c], 1.0e-14);
-    dt[c] = cfl * vol[c] / l;
-    alpha[c] = (l * inv_cfl) + (vol[c] * 0.0);
+    dt[c] = cfl * vol / l;
+    alpha[c] = (l * inv_cfl) + (vol * 0.0);
 }
 
 __device__ __forceinline__ int find_col(const BsrArgs& b, int row, int col) {
@@ -374,25 +452,28 @@ __device__ __forceinline__ int find_col(const BsrArgs& b, int row, int col) {
     return k;  // the pattern holds every face neighbour
 }
 
-// First-order Picard Jacobian, one thread per block row (jacobian_kernel.hpp:74-270):
-// every block of row c is produced by the thread of row c, the diagonal block
-// accumulated over the cell's faces in ascending face order.
+// First-order Picard Jacobian, one thread per block row (jacobian_kernel.hpp:
+// 74-270): all blocks of row c come from the thread of row c; the diagonal
+// block accumulates over the cell's faces in ascending face index.
 __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
                                                      const double* alpha, double* values) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
     double dg[25];
     for (int i = 0; i < 25; ++i) dg[i] = 0.0;
     for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
         const int fc = m.cf[e];
+        const double* fr = frec(m, fc);
         const bool inner = fc < m.ninner;
-        const int c0 = m.owner[fc];
-        const int c1 = inner ? m.neigh[fc] : m.nc + (fc - m.ninner);
-        const double nx = m.fnx[fc], ny = m.fny[fc], nz = m.fnz[fc], A = m.farea[fc];
+        const int c0 = face_owner(fr);
+        const int c1 = inner ? face_neigh(fr) : m.nc + (fc - m.ninner);
+        const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
+        const double* r0 = crec(f, c0);
+        const double* r1 = crec(f, c1);
         double qL[5], qR[5], UL[5], UR[5];
         for (int v = 0; v < 5; ++v) {
-            qL[v] = plane(f.q, m.ld, v)[c0];
-            qR[v] = plane(f.q, m.ld, v)[c1];
+            qL[v] = r0[kRQ + v];
+            qR[v] = r1[kRQ + v];
         }
         prim_to_cons(nm.gas, qL, UL);
         prim_to_cons(nm.gas, qR, UR);
@@ -400,15 +481,14 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, Nu
         const double half_area = 0.5 * A;
         const double diss = 0.5 * smax * A;
         double vm[9];
-        if (nm.viscous) viscous_jacobian_entries(nm.gas, m.dinv[fc], qL, qR, 0.0, 0.0, A, vm);
+        if (nm.viscous) viscous_jacobian_entries(nm.gas, fr[kFDinv], qL, qR, 0.0, 0.0, A, vm);
         double J[25], Jv[25];
         const bool own = (c0 == c);
-        // Left (owner) side: dFL = 0.5 A J(UL) + diss I (+ viscous); right side:
-        // dFR = 0.5 A J(UR) - diss I (- viscous).
-        // Row c needs: diag += dFL (owner) or -= dFR (neighbour); off-diagonal =
-        // dFR (owner) or -dFL (neighbour).
+        // dFL = 0.5 A J(UL) + diss I (+ viscous), dFR = 0.5 A J(UR) - diss I (- viscous);
+        // row c: diagonal += dFL (owner) or -= dFR (neighbour); off-diagonal =
+        // dFR (owner) or -dFL (neighbour)
         for (int pass = 0; pass < (inner ? 2 : 1); ++pass) {
-            const bool left = (pass == 0) ? own : !own;  // pass 0: diagonal term, pass 1: off-diagonal term
+            const bool left = (pass == 0) ? own : !own;
             euler_jacobian_scaled(nm.gas, left ? UL : UR, nx, ny, nz, half_area, J);
             for (int i = 0; i < 5; ++i) J[i * 6] = left ? (J[i * 6] + diss) : (J[i * 6] - diss);
             if (nm.viscous) {
@@ -460,11 +540,12 @@ void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs
 }
 
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_flux, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
+    if (m.ntiles > 0) hipLaunchKernelGGL(k_flux, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
 }
 
-void launch_dt(const MeshArgs& m, const double* lam, double cfl, double* dt, double* alpha, hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, m.vol, lam, cfl, dt, alpha);
+void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
+               hipStream_t s) {
+    if (m.nc > 0) hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.crec, lam, cfl, dt, alpha);
 }
 
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,

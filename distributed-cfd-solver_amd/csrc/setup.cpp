@@ -239,6 +239,84 @@ std::vector<i32> forward_levels(const Csr& ccf, i32 n, i32& n_levels) {
     return lev;
 }
 
+Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, int max_faces) {
+    const i32 n = m.n_cells;
+    Tiling t;
+    t.old2new.assign(z(n), -1);
+    t.new2old.reserve(z(n));
+    t.tile_off.push_back(0);
+    std::vector<i32> queue, members;
+    std::vector<i32> face_mark(z(m.n_faces), -1);
+    i32 tile = 0;
+    for (i32 seed = 0; seed < n; ++seed) {
+        if (t.old2new[z(seed)] >= 0) continue;
+        members.clear();
+        queue.clear();
+        queue.push_back(seed);
+        t.old2new[z(seed)] = -2;  // queued
+        int n_faces = 0;
+        std::size_t head = 0;
+        while (head < queue.size() && static_cast<int>(members.size()) < max_cells) {
+            const i32 c = queue[head++];
+            // faces this cell would add to the tile
+            int add = 0;
+            for (i32 e = cf.off[z(c)]; e < cf.off[z(c) + 1]; ++e)
+                if (face_mark[z(cf.idx[z(e)])] != tile) ++add;
+            if (n_faces + add > max_faces) {
+                t.old2new[z(c)] = -1;  // leave for a later tile
+                continue;
+            }
+            n_faces += add;
+            for (i32 e = cf.off[z(c)]; e < cf.off[z(c) + 1]; ++e) face_mark[z(cf.idx[z(e)])] = tile;
+            members.push_back(c);
+            t.old2new[z(c)] = -3;  // member
+            for (i32 e = ccf.off[z(c)]; e < ccf.off[z(c) + 1]; ++e) {
+                const i32 nb = ccf.idx[z(e)];
+                if (t.old2new[z(nb)] == -1) {
+                    t.old2new[z(nb)] = -2;
+                    queue.push_back(nb);
+                }
+            }
+        }
+        for (std::size_t q = head; q < queue.size(); ++q)
+            if (t.old2new[z(queue[q])] == -2) t.old2new[z(queue[q])] = -1;
+        std::sort(members.begin(), members.end());
+        for (const i32 c : members) {
+            t.old2new[z(c)] = static_cast<i32>(t.new2old.size());
+            t.new2old.push_back(c);
+        }
+        t.tile_off.push_back(static_cast<i32>(t.new2old.size()));
+        ++tile;
+    }
+    if (static_cast<i32>(t.new2old.size()) != n) throw std::runtime_error("tiling: not every cell assigned");
+    return t;
+}
+
+TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>& face_owner_dev) {
+    TileFaces tf;
+    const std::size_t ntiles = t.tile_off.size() - 1;
+    tf.tile_face_off.assign(ntiles + 1, 0);
+    tf.cell_slot.assign(dev_cf.idx.size(), 0);
+    std::vector<i32> local;
+    for (std::size_t k = 0; k < ntiles; ++k) {
+        local.clear();
+        for (i32 c = t.tile_off[k]; c < t.tile_off[k + 1]; ++c)
+            for (i32 e = dev_cf.off[z(c)]; e < dev_cf.off[z(c) + 1]; ++e) local.push_back(dev_cf.idx[z(e)]);
+        std::sort(local.begin(), local.end());
+        local.erase(std::unique(local.begin(), local.end()), local.end());
+        for (i32 c = t.tile_off[k]; c < t.tile_off[k + 1]; ++c) {
+            for (i32 e = dev_cf.off[z(c)]; e < dev_cf.off[z(c) + 1]; ++e) {
+                const i32 f = dev_cf.idx[z(e)];
+                const i32 slot = static_cast<i32>(std::lower_bound(local.begin(), local.end(), f) - local.begin());
+                tf.cell_slot[z(e)] = (face_owner_dev[z(f)] == c) ? slot : -slot - 1;
+            }
+        }
+        tf.faces.insert(tf.faces.end(), local.begin(), local.end());
+        tf.tile_face_off[k + 1] = static_cast<i32>(tf.faces.size());
+    }
+    return tf;
+}
+
 std::vector<i32> backward_levels(const Csr& ccf, i32 n, i32& n_levels) {
     std::vector<i32> lev(z(n), 0);
     n_levels = (n > 0) ? 1 : 0;

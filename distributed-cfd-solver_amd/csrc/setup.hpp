@@ -52,4 +52,27 @@ RowGroups groups_from_labels(const std::vector<std::int32_t>& label, std::int32_
 std::vector<std::int32_t> forward_levels(const Csr& ccf, std::int32_t n, std::int32_t& n_levels);
 std::vector<std::int32_t> backward_levels(const Csr& ccf, std::int32_t n, std::int32_t& n_levels);
 
+// Device cell order: the cells are grouped into compact tiles (greedy
+// breadth-first growth over the face graph, seeds taken in index order) of at
+// most max_cells cells whose faces (interior + boundary, counted once) number
+// at most max_faces.  Inside a tile cells keep ascending index order.
+struct Tiling {
+    std::vector<std::int32_t> new2old;   // device cell -> mesh cell
+    std::vector<std::int32_t> old2new;
+    std::vector<std::int32_t> tile_off;  // tiles + 1, offsets into the device order
+};
+Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, int max_faces);
+
+// Per-tile face lists for the tiled flux kernel, in device numbering:
+//   faces[tile_face_off[t] .. tile_face_off[t+1]) = the faces touching tile t,
+//   ascending face index; cell_slot[e] (parallel to the device cell->faces
+//   list) = position of that face in its tile's list, negated and minus one
+//   when the cell is the face's neighbour (the flux enters with a minus sign).
+struct TileFaces {
+    std::vector<std::int32_t> tile_face_off;
+    std::vector<std::int32_t> faces;
+    std::vector<std::int32_t> cell_slot;
+};
+TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<std::int32_t>& face_owner_dev);
+
 }  // namespace cfdg
