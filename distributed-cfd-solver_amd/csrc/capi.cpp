@@ -168,6 +168,8 @@ struct cfdg_ctx {
     // mesh (device)
     DBuf<double> frec;
     DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
+    DBuf<int> tile_halo_off, tile_halo;
+    DBuf<std::uint16_t> ccn_slot, ccf_slot;
     // fields (device order)
     DBuf<double> U, crec, res, lam, dt, alpha;
     bool have_residual = false;
@@ -210,6 +212,10 @@ struct cfdg_ctx {
         a.tile_off = tile_off.get();
         a.tile_face_off = tile_face_off.get();
         a.tile_faces = tile_faces.get();
+        a.tile_halo_off = tile_halo_off.get();
+        a.tile_halo = tile_halo.get();
+        a.ccn_slot = ccn_slot.get();
+        a.ccf_slot = ccf_slot.get();
         return a;
     }
     cfdg::FieldArgs fargs() const {
@@ -665,6 +671,15 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         const cfdg::Csr st = to_device_rows(cfdg::cell_cells_by_node(m), c.new2old, &c.old2new);
         c.ccn_off.upload(st.off, s);
         c.ccn.upload(st.idx, s);
+        // LDS-staged stencils when every tile's cells + halo fit the kernel's slots
+        const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
+        const cfdg::TileStencil ts = cfdg::tile_stencil(tl, st, ccf_dev);
+        if (ts.max_slots <= cfdg::kGradSlots) {
+            c.tile_halo_off.upload(ts.halo_off, s);
+            c.tile_halo.upload(ts.halo, s);
+            c.ccn_slot.upload(ts.ccn_slot, s);
+            c.ccf_slot.upload(ts.ccf_slot, s);
+        }
         c.sync();
     }
     // cell records: static centroid and volume; everything else zero

@@ -19,6 +19,7 @@ constexpr int kMaxPatches = 64;
 constexpr int kTileCells = 256;       // cells per tile (= threads per flux workgroup)
 constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux kernel
 constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
+constexpr int kGradSlots = 640;       // LDS slots (tile cells + stencil halo) of the staged gradient kernel
 
 // Cell record, 32 doubles (256 B, two 128-B lines), one per cell and per
 // boundary ghost: line 0 = what neighbour gathers need (primitives, centroid,
@@ -53,6 +54,11 @@ struct MeshArgs {
     const int* tile_off;      // tiles + 1
     const int* tile_face_off; // tiles + 1
     const int* tile_faces;
+    // LDS-staged gradient stencils (null when the tiles' stencils do not fit)
+    const int* tile_halo_off; // tiles + 1
+    const int* tile_halo;     // halo cells of each tile, ascending device id
+    const std::uint16_t* ccn_slot;  // LDS slot of each ccn entry
+    const std::uint16_t* ccf_slot;  // LDS slot of each ccf entry
 };
 
 struct FieldArgs {

@@ -28,14 +28,19 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
                                                       std::uint8_t* piv, int* singular) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // fully unrolled so that the 25 entries stay in registers: the row swap is
+    // a select over the (compile-time) candidate rows
     double a[25];
     const double* src = values + static_cast<long long>(diag[i]) * 25;
+#pragma unroll
     for (int k = 0; k < 25; ++k) a[k] = src[k];
-    std::uint8_t p[5];
+    int p[5] = {0, 1, 2, 3, 4};
     bool ok = true;
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
         int mrow = k;
         double mx = fabs(a[k * 5 + k]);
+#pragma unroll
         for (int r = k + 1; r < 5; ++r) {
             const double v = fabs(a[r * 5 + k]);
             if (v > mx) {
@@ -43,22 +48,24 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
                 mrow = r;
             }
         }
-        p[k] = static_cast<std::uint8_t>(mrow);
-        if (mx == 0.0) {
-            ok = false;
-            break;
-        }
-        if (mrow != k) {
+        p[k] = mrow;
+        ok = ok && (mx != 0.0);
+#pragma unroll
+        for (int r = k + 1; r < 5; ++r) {
+            const bool sw = (mrow == r);
+#pragma unroll
             for (int j = 0; j < 5; ++j) {
-                const double t = a[k * 5 + j];
-                a[k * 5 + j] = a[mrow * 5 + j];
-                a[mrow * 5 + j] = t;
+                const double tk = a[k * 5 + j], tr = a[r * 5 + j];
+                a[k * 5 + j] = sw ? tr : tk;
+                a[r * 5 + j] = sw ? tk : tr;
             }
         }
         const double inv_d = 1.0 / a[k * 5 + k];
+#pragma unroll
         for (int r = k + 1; r < 5; ++r) {
             const double l = a[r * 5 + k] * inv_d;
             a[r * 5 + k] = l;
+#pragma unroll
             for (int j = k + 1; j < 5; ++j) a[r * 5 + j] -= l * a[k * 5 + j];
         }
     }
@@ -67,8 +74,10 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
         return;
     }
     double* dst = lu + static_cast<long long>(i) * 25;
+#pragma unroll
     for (int k = 0; k < 25; ++k) dst[k] = a[k];
-    for (int k = 0; k < 5; ++k) piv[static_cast<long long>(i) * 5 + k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) piv[static_cast<long long>(i) * 5 + k] = static_cast<std::uint8_t>(p[k]);
 }
 
 // y = A x with five lanes per block row: lane ii forms
@@ -235,13 +244,17 @@ __global__ void __launch_bounds__(kBlock) k_res_sq(int nc, int ld, const double*
     }
 }
 
-// Adds nblocks partial 5-vectors in block order (one thread per component).
-__global__ void k_reduce_partials(int nblocks, const double* partial, double* out) {
-    const int j = threadIdx.x;
-    if (j >= 5) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partial[b * 5 + j];
-    out[j] = s;
+// Adds nblocks partial 5-vectors: thread t sums blocks t, t + 256, ... in
+// order, then the block tree adds the 256 thread sums in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_reduce_partials(int nblocks, int ncomp, const double* partial,
+                                                            double* out) {
+    __shared__ double lds[kBlock / 64];
+    for (int j = 0; j < ncomp; ++j) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b * 5 + j];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) out[j] = s;
+    }
 }
 
 // b = -R (AoS), the implicit right-hand side (implicit_euler.hpp:81-92)
@@ -304,13 +317,13 @@ int reduction_blocks(long long n) {
 void launch_dots(long long n, int ndot, const DotPairs& pr, double* partial, double* out, hipStream_t s) {
     const int nb = reduction_blocks(n);
     hipLaunchKernelGGL(k_dots, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, nb, partial, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out);
 }
 
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s) {
     const int nb = reduction_blocks(nc);
     hipLaunchKernelGGL(k_res_sq, dim3(nb), dim3(kBlock), 0, s, nc, ld, res, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, nb, partial, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, 5, partial, out);
 }
 
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s) {

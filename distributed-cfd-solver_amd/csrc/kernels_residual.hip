@@ -8,7 +8,8 @@
 //   k_gradient_limiter  cell gradient (node-stencil least squares with the
 //                       3x3 normal equations rebuilt per cell from neighbour
 //                       centroids, or Green-Gauss) + MUSCL limiter, one thread
-//                       per cell
+//                       per cell; k_gradient_limiter_lds is the same per cell
+//                       with the tile's neighbour stencil staged in LDS
 //   k_ghost_gradients   boundary ghost gradients
 //   k_flux              tiled flux residual: a workgroup owns a tile of cells,
 //                       evaluates each face touching the tile once into LDS,
@@ -126,15 +127,18 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
     for (int v = 0; v < 5; ++v) rg[kRQ + v] = qg[v];
 }
 
-// Cell gradient + MUSCL limiter (gradient_operator.cpp:549-707, :197-278;
-// muscl.hpp:26-104), one thread per cell.
-__global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldArgs f, NumArgs nm) {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (c >= m.nc) return;
+// Cell gradient + MUSCL limiter of device cell c (gradient_operator.cpp:549-707,
+// :197-278; muscl.hpp:26-104).  `self`, nbn(j) and nbf(j) point at 8 doubles
+// laid out as the first half of a cell record (q, centroid) for the cell, its
+// j-th vertex neighbour and its j-th face neighbour -- either the records in
+// HBM or their copies staged in LDS.
+template <class NbN, class NbF>
+__device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
+                                                      const double* self, NbN nbn, NbF nbf) {
     double* rc = crec(f, c);
     double qc[5];
-    for (int v = 0; v < 5; ++v) qc[v] = rc[kRQ + v];
-    const double x0 = rc[kRC], y0 = rc[kRC + 1], z0 = rc[kRC + 2];
+    for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
+    const double x0 = self[kRC], y0 = self[kRC + 1], z0 = self[kRC + 2];
     double g[5][3];
     for (int v = 0; v < 5; ++v) g[v][0] = g[v][1] = g[v][2] = 0.0;
 
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
         if (je > jb) {
             double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
             for (int j = jb; j < je; ++j) {
-                const double* rn = crec(f, m.ccn[j]);
+                const double* rn = nbn(j);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
                 inv[8] = (A0 * A4 - A1 * A3) * id;
             }
             for (int j = jb; j < je; ++j) {
-                const double* rn = crec(f, m.ccn[j]);
+                const double* rn = nbn(j);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
@@ -235,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
     double qmax[5], qmin[5];
     for (int v = 0; v < 5; ++v) qmax[v] = qmin[v] = qc[v];
     for (int j = m.ccf_off[c]; j < m.ccf_off[c + 1]; ++j) {
-        const double* rn = crec(f, m.ccf[j]);
+        const double* rn = nbf(j);
         for (int v = 0; v < 5; ++v) {
             const double qn = rn[kRQ + v];
             qmax[v] = dmax(qmax[v], qn);
@@ -254,6 +258,41 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
         }
     }
     for (int v = 0; v < 5; ++v) rc[kRPhi + v] = ph[v];
+}
+
+// One thread per cell, neighbours gathered from the records in HBM.
+__global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldArgs f, NumArgs nm) {
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    gradient_limiter_cell(
+        m, f, nm, c, crec(f, c), [&](int j) { return static_cast<const double*>(crec(f, m.ccn[j])); },
+        [&](int j) { return static_cast<const double*>(crec(f, m.ccf[j])); });
+}
+
+// One workgroup per tile: the (q, centroid) halves of the records of the
+// tile's cells and of its stencil halo are staged once into LDS, then every
+// cell reads its vertex and face neighbours from there.
+__global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm) {
+    __shared__ double sq[kGradSlots * 8];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
+    const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
+    for (int sl = threadIdx.x; sl < ns; sl += kTileCells) {
+        const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
+        const double2* r = reinterpret_cast<const double2*>(crec(f, src));
+        double2* d = reinterpret_cast<double2*>(sq + sl * 8);
+        const double2 a = r[0], b = r[1], cc = r[2], e = r[3];
+        d[0] = a;
+        d[1] = b;
+        d[2] = cc;
+        d[3] = e;
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) >= nt) return;
+    gradient_limiter_cell(
+        m, f, nm, c0 + threadIdx.x, sq + threadIdx.x * 8,
+        [&](int j) { return static_cast<const double*>(sq + m.ccn_slot[j] * 8); },
+        [&](int j) { return static_cast<const double*>(sq + m.ccf_slot[j] * 8); });
 }
 
 __device__ __forceinline__ void grad_dirichlet(double* g, const double* gi, double qin, double qfix, double nx,
@@ -530,7 +569,12 @@ void launch_ghosts(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, con
 }
 
 void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
+    if (m.nc <= 0) return;
+    if (m.tile_halo_off != nullptr)
+        hipLaunchKernelGGL(k_gradient_limiter_lds, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f,
+                           nm);
+    else
+        hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
 }
 
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,

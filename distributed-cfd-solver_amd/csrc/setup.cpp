@@ -239,56 +239,96 @@ std::vector<i32> forward_levels(const Csr& ccf, i32 n, i32& n_levels) {
     return lev;
 }
 
+namespace {
+
+// Faces touching cells[lo, hi), each counted once.
+int tile_face_count(const Csr& cf, const Mesh& m, const std::vector<i32>& cells, std::size_t lo, std::size_t hi,
+                    std::vector<i32>& stamp, i32 tag) {
+    for (std::size_t i = lo; i < hi; ++i) stamp[z(cells[i])] = tag;
+    int n = 0;
+    for (std::size_t i = lo; i < hi; ++i) {
+        const i32 c = cells[i];
+        for (i32 e = cf.off[z(c)]; e < cf.off[z(c) + 1]; ++e) {
+            const i32 f = cf.idx[z(e)];
+            const i32 o = m.face_owner[z(f)], nb = m.face_neigh[z(f)];
+            const i32 other = (o == c) ? nb : o;
+            // an interior face with both cells in the set is counted at its owner only
+            if (other >= 0 && stamp[z(other)] == tag && o != c) continue;
+            ++n;
+        }
+    }
+    return n;
+}
+
+struct Bisector {
+    const Mesh& m;
+    const Csr& ccf;
+    const Csr& cf;
+    int max_cells, max_faces;
+    std::vector<i32> cells, work, best;
+    std::vector<i32> stamp;  // per cell: tag of the set it was last marked in
+    i32 tag = 0;
+    std::vector<i32> tile_off;
+
+    double coord(i32 c, int a) const {
+        return a == 0 ? m.cell_cx[z(c)] : (a == 1 ? m.cell_cy[z(c)] : m.cell_cz[z(c)]);
+    }
+
+    void split(std::size_t lo, std::size_t hi) {
+        const std::size_t n = hi - lo;
+        const std::size_t cap = static_cast<std::size_t>(max_cells);
+        if (n <= cap && tile_face_count(cf, m, cells, lo, hi, stamp, ++tag) <= max_faces) {
+            tile_off.push_back(static_cast<i32>(hi));
+            return;
+        }
+        if (n < 2) throw std::runtime_error("tiling: one cell has more faces than a tile holds");
+        const std::size_t parts = std::max<std::size_t>(2, (n + cap - 1) / cap);
+        const std::size_t nl = n * (parts / 2) / parts;
+        long long best_cut = -1;
+        for (int a = 0; a < 3; ++a) {
+            work.assign(cells.begin() + static_cast<long>(lo), cells.begin() + static_cast<long>(hi));
+            std::nth_element(work.begin(), work.begin() + static_cast<long>(nl), work.end(), [&](i32 x, i32 y) {
+                const double cx = coord(x, a), cy = coord(y, a);
+                return cx < cy || (cx == cy && x < y);
+            });
+            // faces crossing the split: left cells whose face neighbour is on the right
+            const i32 tl = ++tag, tr = ++tag;
+            for (std::size_t i = 0; i < nl; ++i) stamp[z(work[i])] = tl;
+            for (std::size_t i = nl; i < n; ++i) stamp[z(work[i])] = tr;
+            long long cut = 0;
+            for (std::size_t i = 0; i < nl; ++i) {
+                const i32 c = work[i];
+                for (i32 e = ccf.off[z(c)]; e < ccf.off[z(c) + 1]; ++e)
+                    if (stamp[z(ccf.idx[z(e)])] == tr) ++cut;
+            }
+            if (best_cut < 0 || cut < best_cut) {
+                best_cut = cut;
+                best.swap(work);
+            }
+        }
+        std::copy(best.begin(), best.end(), cells.begin() + static_cast<long>(lo));
+        split(lo, lo + nl);
+        split(lo + nl, hi);
+    }
+};
+
+}  // namespace
+
 Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, int max_faces) {
     const i32 n = m.n_cells;
     Tiling t;
+    Bisector b{m, ccf, cf, max_cells, max_faces, {}, {}, {}, {}, 0, {}};
+    b.cells.resize(z(n));
+    for (i32 c = 0; c < n; ++c) b.cells[z(c)] = c;
+    b.stamp.assign(z(n), 0);
+    b.tile_off.push_back(0);
+    if (n > 0) b.split(0, z(n));
+    t.tile_off = std::move(b.tile_off);
+    t.new2old = std::move(b.cells);
+    for (std::size_t k = 0; k + 1 < t.tile_off.size(); ++k)
+        std::sort(t.new2old.begin() + t.tile_off[k], t.new2old.begin() + t.tile_off[k + 1]);
     t.old2new.assign(z(n), -1);
-    t.new2old.reserve(z(n));
-    t.tile_off.push_back(0);
-    std::vector<i32> queue, members;
-    std::vector<i32> face_mark(z(m.n_faces), -1);
-    i32 tile = 0;
-    for (i32 seed = 0; seed < n; ++seed) {
-        if (t.old2new[z(seed)] >= 0) continue;
-        members.clear();
-        queue.clear();
-        queue.push_back(seed);
-        t.old2new[z(seed)] = -2;  // queued
-        int n_faces = 0;
-        std::size_t head = 0;
-        while (head < queue.size() && static_cast<int>(members.size()) < max_cells) {
-            const i32 c = queue[head++];
-            // faces this cell would add to the tile
-            int add = 0;
-            for (i32 e = cf.off[z(c)]; e < cf.off[z(c) + 1]; ++e)
-                if (face_mark[z(cf.idx[z(e)])] != tile) ++add;
-            if (n_faces + add > max_faces) {
-                t.old2new[z(c)] = -1;  // leave for a later tile
-                continue;
-            }
-            n_faces += add;
-            for (i32 e = cf.off[z(c)]; e < cf.off[z(c) + 1]; ++e) face_mark[z(cf.idx[z(e)])] = tile;
-            members.push_back(c);
-            t.old2new[z(c)] = -3;  // member
-            for (i32 e = ccf.off[z(c)]; e < ccf.off[z(c) + 1]; ++e) {
-                const i32 nb = ccf.idx[z(e)];
-                if (t.old2new[z(nb)] == -1) {
-                    t.old2new[z(nb)] = -2;
-                    queue.push_back(nb);
-                }
-            }
-        }
-        for (std::size_t q = head; q < queue.size(); ++q)
-            if (t.old2new[z(queue[q])] == -2) t.old2new[z(queue[q])] = -1;
-        std::sort(members.begin(), members.end());
-        for (const i32 c : members) {
-            t.old2new[z(c)] = static_cast<i32>(t.new2old.size());
-            t.new2old.push_back(c);
-        }
-        t.tile_off.push_back(static_cast<i32>(t.new2old.size()));
-        ++tile;
-    }
-    if (static_cast<i32>(t.new2old.size()) != n) throw std::runtime_error("tiling: not every cell assigned");
+    for (i32 d = 0; d < n; ++d) t.old2new[z(t.new2old[z(d)])] = d;
     return t;
 }
 
@@ -315,6 +355,48 @@ TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>&
         tf.tile_face_off[k + 1] = static_cast<i32>(tf.faces.size());
     }
     return tf;
+}
+
+TileStencil tile_stencil(const Tiling& t, const Csr& dev_ccn, const Csr& dev_ccf) {
+    TileStencil ts;
+    const std::size_t ntiles = t.tile_off.size() - 1;
+    const std::size_t n = t.new2old.size();
+    ts.halo_off.assign(ntiles + 1, 0);
+    ts.ccn_slot.assign(dev_ccn.idx.size(), 0);
+    ts.ccf_slot.assign(dev_ccf.idx.size(), 0);
+    std::vector<i32> slot(n, -1), local;
+    for (std::size_t k = 0; k < ntiles; ++k) {
+        const i32 c0 = t.tile_off[k], c1 = t.tile_off[k + 1];
+        local.clear();
+        for (i32 c = c0; c < c1; ++c) {
+            for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
+                const i32 nb = dev_ccn.idx[z(e)];
+                if (nb < c0 || nb >= c1) local.push_back(nb);
+            }
+            for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) {
+                const i32 nb = dev_ccf.idx[z(e)];
+                if (nb < c0 || nb >= c1) local.push_back(nb);
+            }
+        }
+        std::sort(local.begin(), local.end());
+        local.erase(std::unique(local.begin(), local.end()), local.end());
+        const i32 nslots = (c1 - c0) + static_cast<i32>(local.size());
+        if (nslots > 65535) throw std::runtime_error("tile stencil: more than 65535 LDS slots");
+        ts.max_slots = std::max(ts.max_slots, nslots);
+        for (i32 c = c0; c < c1; ++c) slot[z(c)] = c - c0;
+        for (std::size_t h = 0; h < local.size(); ++h) slot[z(local[h])] = (c1 - c0) + static_cast<i32>(h);
+        for (i32 c = c0; c < c1; ++c) {
+            for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e)
+                ts.ccn_slot[z(e)] = static_cast<std::uint16_t>(slot[z(dev_ccn.idx[z(e)])]);
+            for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e)
+                ts.ccf_slot[z(e)] = static_cast<std::uint16_t>(slot[z(dev_ccf.idx[z(e)])]);
+        }
+        for (i32 c = c0; c < c1; ++c) slot[z(c)] = -1;
+        for (const i32 h : local) slot[z(h)] = -1;
+        ts.halo.insert(ts.halo.end(), local.begin(), local.end());
+        ts.halo_off[k + 1] = static_cast<i32>(ts.halo.size());
+    }
+    return ts;
 }
 
 std::vector<i32> backward_levels(const Csr& ccf, i32 n, i32& n_levels) {

@@ -52,10 +52,15 @@ RowGroups groups_from_labels(const std::vector<std::int32_t>& label, std::int32_
 std::vector<std::int32_t> forward_levels(const Csr& ccf, std::int32_t n, std::int32_t& n_levels);
 std::vector<std::int32_t> backward_levels(const Csr& ccf, std::int32_t n, std::int32_t& n_levels);
 
-// Device cell order: the cells are grouped into compact tiles (greedy
-// breadth-first growth over the face graph, seeds taken in index order) of at
-// most max_cells cells whose faces (interior + boundary, counted once) number
-// at most max_faces.  Inside a tile cells keep ascending index order.
+// Device cell order: the cells are grouped into compact tiles by recursive
+// bisection of the centroids.  Each split cuts the set at the median of one
+// coordinate axis, choosing the axis whose median plane crosses the fewest
+// faces (on stretched boundary-layer cells this splits along the direction
+// with the most cells, not the longest one).  A set becomes a tile once it
+// holds at most max_cells cells and at most max_faces faces (interior +
+// boundary, counted once).  Tiles come out in depth-first order of the
+// bisection tree, so consecutive tiles, and any contiguous range of tiles,
+// are spatially compact.  Inside a tile cells keep ascending index order.
 struct Tiling {
     std::vector<std::int32_t> new2old;   // device cell -> mesh cell
     std::vector<std::int32_t> old2new;
@@ -74,5 +79,20 @@ struct TileFaces {
     std::vector<std::int32_t> cell_slot;
 };
 TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<std::int32_t>& face_owner_dev);
+
+// LDS-staged neighbour stencils for the gradient kernel, in device numbering.
+// A tile's LDS slots are its own cells (slot = cell - tile start) followed by
+// halo[halo_off[t] .. halo_off[t+1]): the cells outside the tile that its
+// stencils reach, ascending.  ccn_slot / ccf_slot give, for every entry of the
+// device vertex-neighbour / face-neighbour lists, the neighbour's slot in the
+// LDS of the tile that owns the row.  max_slots is the largest tile + halo.
+struct TileStencil {
+    std::vector<std::int32_t> halo_off;
+    std::vector<std::int32_t> halo;
+    std::vector<std::uint16_t> ccn_slot;
+    std::vector<std::uint16_t> ccf_slot;
+    std::int32_t max_slots = 0;
+};
+TileStencil tile_stencil(const Tiling& t, const Csr& dev_ccn, const Csr& dev_ccf);
 
 }  // namespace cfdg
