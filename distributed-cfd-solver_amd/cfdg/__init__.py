@@ -85,6 +85,31 @@ class StepResult(C.Structure):
     _fields_ = [("l2", _d * 5), ("linear", IterResult)]
 
 
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+CONVERGED, DIVERGED, MAX_ITERATIONS = 0, 1, 2
+
+
+class SteadyParams(C.Structure):
+    _fields_ = [("max_steps", _i64), ("rel_tol", _d), ("check_interval", _i32)]
+
+
+class SteadyResult(C.Structure):
+    _fields_ = [("status", _i32), ("steps", _i64), ("linear_iterations", _i64), ("rel", _d),
+                ("l2_first", _d * 5), ("l2_last", _d * 5), ("seconds", _d)]
+
+
+class Precision(C.Structure):
+    """cfdg_precision: block storage, Krylov precision, defect-correction controls."""
+    _fields_ = [("block_storage", _i32), ("krylov", _i32), ("max_outer", _i32), ("reserved", _i32),
+                ("inner_rel_tol", _d)]
+
+    @classmethod
+    def make(cls, block_storage=PREC_FP64, krylov=PREC_FP64, max_outer=8, inner_rel_tol=0.1):
+        p = cls()
+        p.block_storage, p.krylov, p.max_outer, p.reserved, p.inner_rel_tol = block_storage, krylov, max_outer, 0, inner_rel_tol
+        return p
+
+
 # Mesh array fields of MeshView: name -> (dtype, length attribute)
 MESH_ARRAYS = {
     "cell_type": (np.uint8, "n_cells"), "cell_nodes_off": (np.int32, "n_cells+1"), "cell_nodes": (np.int32, None),
@@ -116,6 +141,15 @@ def _load():
         "cfdg_get_bsr_values": [vp, vp], "cfdg_set_bsr_values": [vp, vp],
         "cfdg_linear_solve": [vp, vp, vp, vp, vp, vp], "cfdg_implicit_step": [vp, vp, vp],
         "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
+        "cfdg_set_precision": [vp, vp], "cfdg_get_precision": [vp, vp], "cfdg_run_steady": [vp, vp, vp, vp],
+        "cfdg_boundary_integrals": [vp, vp, vp, vp, _i32],
+        "cfdg_gnn_create": [C.c_uint64, vp], "cfdg_gnn_destroy": [vp], "cfdg_gnn_num_params": [vp],
+        "cfdg_gnn_get_params": [vp, vp, _i32], "cfdg_gnn_set_params": [vp, vp, _i32],
+        "cfdg_set_freestream": [vp, _d, vp, _d], "cfdg_gnn_set_recording": [vp, _i32], "cfdg_gnn_record": [vp, vp],
+        "cfdg_gnn_num_samples": [vp, vp], "cfdg_gnn_train": [vp, _i32, C.c_float, C.c_uint64, vp],
+        "cfdg_gnn_loss_grad": [vp, _i32, vp, vp, _i32],
+        "cfdg_gnn_get_sample": [vp, _i32, vp, vp, vp, vp, vp, vp, vp, vp], "cfdg_gnn_predict": [vp, vp, vp],
+        "cfdg_set_warm_start": [vp, vp], "cfdg_warm_start_stats": [vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -125,6 +159,8 @@ def _load():
     lib.cfdg_mesh_free.restype = None
     lib.cfdg_last_error.argtypes = [vp]
     lib.cfdg_last_error.restype = C.c_char_p
+    lib.cfdg_gnn_last_error.argtypes = [vp]
+    lib.cfdg_gnn_last_error.restype = C.c_char_p
     return lib
 
 
@@ -355,6 +391,44 @@ class Solver:
         self._chk(lib.cfdg_implicit_step(self._h, C.byref(lp), C.byref(out)))
         return out
 
+    def boundary_integrals(self):
+        """Per patch: net mass flux, energy flux, force (momentum flux, 3) of the last residual."""
+        n = self.mesh.n_patches
+        mass, energy, force = np.empty(n), np.empty(n), np.empty((n, 3))
+        self._chk(lib.cfdg_boundary_integrals(self._h, _ptr(mass), _ptr(energy), _ptr(force), n))
+        return mass, energy, force
+
+    def run_steady(self, lp, max_steps, rel_tol=1e-8, check_interval=1):
+        sp = SteadyParams()
+        sp.max_steps, sp.rel_tol, sp.check_interval = max_steps, rel_tol, check_interval
+        out = SteadyResult()
+        self._chk(lib.cfdg_run_steady(self._h, C.byref(lp), C.byref(sp), C.byref(out)))
+        return out
+
+    def set_freestream(self, rho, velocity, p):
+        v = (_d * 3)(*velocity)
+        self._chk(lib.cfdg_set_freestream(self._h, rho, v, p))
+
+    def set_recording(self, on=True):
+        self._chk(lib.cfdg_gnn_set_recording(self._h, 1 if on else 0))
+
+    def set_warm_start(self, gnn):
+        self._chk(lib.cfdg_set_warm_start(self._h, gnn._h if gnn is not None else None))
+        self._warm = gnn  # keep the network alive while attached
+
+    def warm_start_stats(self):
+        u, r = _i64(), _i64()
+        self._chk(lib.cfdg_warm_start_stats(self._h, C.byref(u), C.byref(r)))
+        return u.value, r.value
+
+    def set_precision(self, prec):
+        self._chk(lib.cfdg_set_precision(self._h, C.byref(prec)))
+
+    def get_precision(self):
+        p = Precision()
+        self._chk(lib.cfdg_get_precision(self._h, C.byref(p)))
+        return p
+
     def stage_times(self):
         ms = np.empty(6)
         self._chk(lib.cfdg_stage_times(self._h, _ptr(ms)))
@@ -364,3 +438,76 @@ class Solver:
         ms = _d()
         self._chk(lib.cfdg_time_flux_kernel(self._h, reps, C.byref(ms)))
         return ms.value
+
+
+class Gnn:
+    """Learned warm start network (cfdg_gnn_*)."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        rc = lib.cfdg_gnn_create(seed, C.byref(h))
+        if rc != 0:
+            raise CfdgError(rc, "cfdg_gnn_create failed")
+        self._h = h
+        n = _i32()
+        lib.cfdg_gnn_num_params(C.byref(n))
+        self.num_params = n.value
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cfdg_gnn_destroy(self._h)
+            self._h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise CfdgError(rc, lib.cfdg_gnn_last_error(self._h).decode())
+
+    def params(self):
+        p = np.empty(self.num_params, dtype=np.float32)
+        self._chk(lib.cfdg_gnn_get_params(self._h, _ptr(p), self.num_params))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(lib.cfdg_gnn_set_params(self._h, _ptr(p), p.size))
+
+    def record(self, solver):
+        self._chk(lib.cfdg_gnn_record(self._h, solver._h))
+
+    def num_samples(self):
+        n = _i32()
+        self._chk(lib.cfdg_gnn_num_samples(self._h, C.byref(n)))
+        return n.value
+
+    def train(self, epochs, lr=1e-3, seed=0):
+        loss = _d()
+        self._chk(lib.cfdg_gnn_train(self._h, epochs, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def loss_grad(self, sample):
+        g = np.empty(self.num_params, dtype=np.float32)
+        loss = _d()
+        self._chk(lib.cfdg_gnn_loss_grad(self._h, sample, C.byref(loss), _ptr(g), self.num_params))
+        return loss.value, g
+
+    def sample(self, i):
+        n, e = _i32(), _i32()
+        self._chk(lib.cfdg_gnn_get_sample(self._h, i, C.byref(n), C.byref(e), None, None, None, None, None, None))
+        n, e = n.value, e.value
+        x = np.empty((n, 32), np.float32)
+        t = np.empty((n, 5), np.float32)
+        off = np.empty(n + 1, np.int32)
+        adj = np.empty(e, np.int32)
+        rev = np.empty(e, np.int32)
+        eg = np.empty((e, 4), np.float32)
+        self._chk(lib.cfdg_gnn_get_sample(self._h, i, None, None, _ptr(x), _ptr(t), _ptr(off), _ptr(adj), _ptr(rev),
+                                          _ptr(eg)))
+        return {"x": x, "t": t, "adj_off": off, "adj": adj, "rev": rev, "eg": eg}
+
+    def predict(self, solver):
+        du = np.empty(5 * solver.n_cells)
+        self._chk(lib.cfdg_gnn_predict(self._h, solver._h, _ptr(du)))
+        return du

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "ctx.hpp"
 #include "kernels.hpp"
 #include "kernels_linalg.hpp"
 #include "mesh.hpp"
@@ -26,44 +28,10 @@
 namespace {
 
 using cfdg::CfdgError;
+using cfdg::DBuf;
+using cfdg::hip_check;
+using cfdg::z;
 using i32 = std::int32_t;
-inline std::size_t z(long long v) { return static_cast<std::size_t>(v); }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw CfdgError(CFDG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename T>
-class DBuf {
-  public:
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void alloc(std::size_t n) {
-        release();
-        n_ = n;
-        if (n > 0) hip_check(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
-    }
-    void upload(const std::vector<T>& h, hipStream_t s) {
-        alloc(h.size());
-        if (n_ > 0) hip_check(hipMemcpyAsync(p_, h.data(), n_ * sizeof(T), hipMemcpyHostToDevice, s), "upload");
-    }
-    void zero(hipStream_t s) {
-        if (n_ > 0) hip_check(hipMemsetAsync(p_, 0, n_ * sizeof(T), s), "memset");
-    }
-    T* get() const { return p_; }
-    std::size_t size() const { return n_; }
-
-  private:
-    void release() {
-        if (p_) (void)hipFree(p_);
-        p_ = nullptr;
-        n_ = 0;
-    }
-    T* p_ = nullptr;
-    std::size_t n_ = 0;
-};
 
 // Resolved boundary parameters (bc_manager.hpp:235-352 builders, resolve_inflow :37-111).
 cfdg::BcDev resolve_bc(const cfdg_bc& d, double gamma, double R) {
@@ -156,119 +124,6 @@ cfdg::Csr to_device_rows(const cfdg::Csr& in, const std::vector<i32>& new2old, c
 
 }  // namespace
 
-struct cfdg_ctx {
-    std::string err;
-    hipStream_t stream = nullptr;
-    int nc = 0, nf = 0, ninner = 0, nbf = 0, ntot = 0, npatch = 0, ntiles = 0;
-    cfdg_numerics num{};
-    cfdg::NumArgs nargs{};
-    cfdg::BcTable bcs{};
-    std::vector<i32> new2old, old2new;
-
-    // mesh (device)
-    DBuf<double> frec;
-    DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
-    DBuf<int> tile_halo_off, tile_halo;
-    DBuf<std::uint16_t> ccn_slot, ccf_slot;
-    // fields (device order)
-    DBuf<double> U, crec, res, lam, dt, alpha;
-    bool have_residual = false;
-    bool have_dt = false;
-    // BSR: mesh-order pattern for the ABI, device-order pattern for the kernels
-    std::vector<i32> h_row_ptr, h_cols, h_diag;
-    std::vector<long long> blk_dev;  // mesh block k -> device block
-    DBuf<int> row_ptr, cols, diag;
-    DBuf<double> values, lu;
-    DBuf<std::uint8_t> piv;
-    DBuf<int> singular;
-    // SGS row groups (device rows): 0 multicolour fwd, 1 multicolour bwd, 2 sequential fwd, 3 sequential bwd
-    std::vector<i32> h_gofs[4];
-    DBuf<int> grows[4];
-    cfdg::Csr ccf_mesh;
-    bool have_seq_groups = false;
-    // Krylov work vectors (AoS, 5 nc, device order)
-    DBuf<double> b, x, r, r0, p, v, s, t, ph, sh;
-    DBuf<double> partial, dots;
-    double* h_dots = nullptr;  // pinned
-    // timing
-    hipEvent_t ev[8] = {};
-    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
-
-    cfdg::MeshArgs margs() const {
-        cfdg::MeshArgs a{};
-        a.nc = nc;
-        a.nf = nf;
-        a.ninner = ninner;
-        a.ntiles = ntiles;
-        a.frec = frec.get();
-        a.cf_off = cf_off.get();
-        a.cf = cf.get();
-        a.cf_slot = cf_slot.get();
-        a.ccf_off = ccf_off.get();
-        a.ccf = ccf.get();
-        a.ccn_off = ccn_off.get();
-        a.ccn = ccn.get();
-        a.bface_bc = bface_bc.get();
-        a.tile_off = tile_off.get();
-        a.tile_face_off = tile_face_off.get();
-        a.tile_faces = tile_faces.get();
-        a.tile_halo_off = tile_halo_off.get();
-        a.tile_halo = tile_halo.get();
-        a.ccn_slot = ccn_slot.get();
-        a.ccf_slot = ccf_slot.get();
-        return a;
-    }
-    cfdg::FieldArgs fargs() const {
-        cfdg::FieldArgs f{};
-        f.U = U.get();
-        f.crec = crec.get();
-        f.res = res.get();
-        f.lam = lam.get();
-        return f;
-    }
-    cfdg::BsrArgs bargs() const {
-        cfdg::BsrArgs a{};
-        a.n = nc;
-        a.row_ptr = row_ptr.get();
-        a.cols = cols.get();
-        a.diag = diag.get();
-        return a;
-    }
-    void sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
-    void check_launch() { hip_check(hipGetLastError(), "kernel launch"); }
-    template <typename T>
-    void d2h(T* dst, const T* src, std::size_t n) {
-        if (n) hip_check(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream), "D2H");
-    }
-    template <typename T>
-    void h2d(T* dst, const T* src, std::size_t n) {
-        if (n) hip_check(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream), "H2D");
-    }
-    // AoS block vector (5 per cell): mesh order <-> device order
-    std::vector<double> vec_to_device(const double* x_mesh) const {
-        std::vector<double> d(z(nc) * 5);
-        for (int r = 0; r < nc; ++r)
-            for (int k = 0; k < 5; ++k) d[z(r) * 5 + z(k)] = x_mesh[z(new2old[z(r)]) * 5 + z(k)];
-        return d;
-    }
-    void vec_to_mesh(const std::vector<double>& d, double* x_mesh) const {
-        for (int r = 0; r < nc; ++r)
-            for (int k = 0; k < 5; ++k) x_mesh[z(new2old[z(r)]) * 5 + z(k)] = d[z(r) * 5 + z(k)];
-    }
-    std::vector<double> download_records() {
-        std::vector<double> h(z(ntot) * cfdg::kRec);
-        d2h(h.data(), crec.get(), h.size());
-        sync();
-        return h;
-    }
-
-    ~cfdg_ctx() {
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (h_dots) (void)hipHostFree(h_dots);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -307,6 +162,7 @@ void build_seq_groups(cfdg_ctx& c) {
 }
 
 void eval_residual(cfdg_ctx& c) {
+    c.have_features = false;
     const auto m = c.margs();
     const auto f = c.fargs();
     hip_check(hipEventRecord(c.ev[0], c.stream), "event");
@@ -345,7 +201,8 @@ void compute_dt(cfdg_ctx& c) {
 void assemble_jacobian(cfdg_ctx& c) {
     if (!c.have_dt) throw CfdgError(CFDG_ERR_STATE, "assemble_jacobian before compute_dt");
     hip_check(hipEventRecord(c.ev[5], c.stream), "event");
-    cfdg::launch_jacobian(c.margs(), c.fargs(), c.nargs, c.bargs(), c.alpha.get(), c.values.get(), c.stream);
+    cfdg::launch_jacobian(c.margs(), c.fargs(), c.nargs, c.bargs(), c.alpha.get(), c.values.get(), c.storage,
+                          c.stream);
     hip_check(hipEventRecord(c.ev[6], c.stream), "event");
     c.check_launch();
     c.sync();
@@ -359,8 +216,9 @@ void fetch_dots(cfdg_ctx& c, int n) {
     c.sync();
 }
 
-double dot_device(cfdg_ctx& c, const double* a, const double* b) {
-    cfdg::DotPairs pr{};
+template <class T>
+double dot_device(cfdg_ctx& c, const T* a, const T* b) {
+    cfdg::DotPairsT<T> pr{};
     pr.a[0] = a;
     pr.b[0] = b;
     cfdg::launch_dots(5LL * c.nc, 1, pr, c.partial.get(), c.dots.get(), c.stream);
@@ -373,23 +231,45 @@ struct Precond {
     int order = CFDG_SGS_MULTICOLOUR;
 };
 
-// z = M^-1 r: z = 0, forward groups, backward groups (preconditioners.cpp:171-194)
-void precond_apply(cfdg_ctx& c, const Precond& P, const double* r, double* z) {
+// Work vectors of one Krylov precision.
+template <class T>
+struct KVecs {
+    T *r, *r0, *p, *v, *s, *t, *ph, *sh;
+};
+KVecs<double> kvecs64(cfdg_ctx& c) {
+    return {c.r.get(), c.r0.get(), c.p.get(), c.v.get(), c.s.get(), c.t.get(), c.ph.get(), c.sh.get()};
+}
+KVecs<float> kvecs32(cfdg_ctx& c) {
+    auto& f = c.f32;
+    return {f.r.get(), f.r0.get(), f.p.get(), f.v.get(), f.s.get(), f.t.get(), f.ph.get(), f.sh.get()};
+}
+
+template <class T>
+void spmv_stored(cfdg_ctx& c, const T* x, T* y) {
+    cfdg::launch_spmv<T>(c.bargs(), c.values.get(), c.storage, x, y, c.stream);
+}
+
+// z = M^-1 r: z = 0, forward groups, backward groups (preconditioners.cpp:171-194).
+// The backward sweep starts with the group the forward sweep ended with: those
+// rows read only neighbours that the forward sweep already left final, so their
+// backward update equals their forward one bit for bit and is skipped.
+template <class T>
+void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z) {
     const long long n5 = 5LL * c.nc;
     if (P.kind == CFDG_PRECOND_NONE) {
-        hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+        hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(T), hipMemcpyDeviceToDevice, c.stream),
                   "copy");
         return;
     }
-    hip_check(hipMemsetAsync(z, 0, static_cast<std::size_t>(n5) * sizeof(double), c.stream), "memset");
+    hip_check(hipMemsetAsync(z, 0, static_cast<std::size_t>(n5) * sizeof(T), c.stream), "memset");
     const int gf = (P.order == CFDG_SGS_SEQUENTIAL) ? 2 : 0;
     const auto b = c.bargs();
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int gi = gf + sweep;
         const auto& off = c.h_gofs[gi];
-        for (std::size_t k = 0; k + 1 < off.size(); ++k)
-            cfdg::launch_sgs_group(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.lu.get(),
-                                   c.piv.get(), r, z, c.stream);
+        for (std::size_t k = (sweep == 1 ? 1 : 0); k + 1 < off.size(); ++k)
+            cfdg::launch_sgs_group<T>(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.storage,
+                                      c.lu.get(), c.piv.get(), r, z, c.stream);
     }
 }
 
@@ -397,7 +277,8 @@ void precond_setup(cfdg_ctx& c, const Precond& P) {
     if (P.kind != CFDG_PRECOND_SGS) return;
     if (P.order == CFDG_SGS_SEQUENTIAL) build_seq_groups(c);
     c.singular.zero(c.stream);
-    cfdg::launch_lu_factor(c.nc, c.diag.get(), c.values.get(), c.lu.get(), c.piv.get(), c.singular.get(), c.stream);
+    cfdg::launch_lu_factor(c.nc, c.diag.get(), c.values.get(), c.storage, c.lu.get(), c.lu_f32, c.piv.get(),
+                           c.singular.get(), c.stream);
     c.check_launch();
     int sing = 0;
     c.d2h(&sing, c.singular.get(), 1);
@@ -407,27 +288,41 @@ void precond_setup(cfdg_ctx& c, const Precond& P) {
                         "SGS: singular diagonal block at row " + std::to_string(c.new2old[z(sing - 1)]));
 }
 
-// Right-preconditioned BiCGSTAB (bicgstab.cpp:42-201) on device vectors; x holds
-// the initial guess on entry and the solution on exit.
-cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& P, double* x, const double* bvec) {
+// Right-preconditioned BiCGSTAB (bicgstab.cpp:42-201) on device vectors of
+// type T (FP64: the reference's arithmetic; FP32: the inner solve of the
+// defect correction).  x holds the initial guess on entry and the solution on
+// exit.  Scalars and dot products are FP64.
+template <class T>
+cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& P, const KVecs<T>& w, T* x,
+                          const T* bvec, bool zero_fallback = false) {
     const long long N = 5LL * c.nc;
-    const std::size_t bytes = static_cast<std::size_t>(N) * sizeof(double);
+    const std::size_t bytes = static_cast<std::size_t>(N) * sizeof(T);
     cfdg_iter_result res{CFDG_MAX_ITERATIONS, 0, 0.0, 0.0};
-    const auto spmv = [&](const double* in, double* out) { cfdg::launch_spmv(c.bargs(), c.values.get(), in, out, c.stream); };
 
-    spmv(x, c.v.get());
-    cfdg::launch_axpy_copy(N, bvec, -1.0, c.v.get(), c.r.get(), c.stream);
+    spmv_stored<T>(c, x, w.v);
+    cfdg::launch_axpy_copy<T>(N, bvec, -1.0, w.v, w.r, c.stream);
     {
-        cfdg::DotPairs pr{};
+        cfdg::DotPairsT<T> pr{};
         pr.a[0] = bvec;
         pr.b[0] = bvec;
-        pr.a[1] = c.r.get();
-        pr.b[1] = c.r.get();
+        pr.a[1] = w.r;
+        pr.b[1] = w.r;
         cfdg::launch_dots(N, 2, pr, c.partial.get(), c.dots.get(), c.stream);
         fetch_dots(c, 2);
     }
     const double bnorm = std::sqrt(c.h_dots[0]);
     double rnorm = std::sqrt(c.h_dots[1]);
+    if (zero_fallback) {
+        // learned initial guess: keep it only if it lowers the initial residual
+        if (!(rnorm <= bnorm)) {
+            hip_check(hipMemsetAsync(x, 0, bytes, c.stream), "memset");
+            hip_check(hipMemcpyAsync(w.r, bvec, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
+            rnorm = bnorm;
+            ++c.warm_rejected;
+        } else {
+            ++c.warm_used;
+        }
+    }
     const double scale = (bnorm > 0.0) ? bnorm : 1.0;
     const double tol = std::max(lp.abs_tol, lp.rel_tol * scale);
     res.initial_residual = rnorm;
@@ -438,8 +333,8 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
     }
     int it = 0;
     if (res.status == CFDG_MAX_ITERATIONS) {
-        hip_check(hipMemcpyAsync(c.r0.get(), c.r.get(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
-        hip_check(hipMemsetAsync(c.v.get(), 0, bytes, c.stream), "memset");
+        hip_check(hipMemcpyAsync(w.r0, w.r, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
+        hip_check(hipMemsetAsync(w.v, 0, bytes, c.stream), "memset");
         double rho = rnorm * rnorm, rho_old = 1.0, alpha = 1.0, omega = 1.0;
         while (true) {
             if (it >= lp.max_iter) {
@@ -448,30 +343,30 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
             }
             ++it;
             if (it == 1) {
-                hip_check(hipMemcpyAsync(c.p.get(), c.r.get(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
+                hip_check(hipMemcpyAsync(w.p, w.r, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
             } else {
                 const double beta = (rho / rho_old) * (alpha / omega);
                 if (!std::isfinite(beta)) {
                     res.status = CFDG_DIVERGED;
                     break;
                 }
-                cfdg::launch_update_p(N, beta, -beta * omega, c.r.get(), c.v.get(), c.p.get(), c.stream);
+                cfdg::launch_update_p<T>(N, beta, -beta * omega, w.r, w.v, w.p, c.stream);
             }
-            precond_apply(c, P, c.p.get(), c.ph.get());
-            spmv(c.ph.get(), c.v.get());
-            const double gamma = dot_device(c, c.r0.get(), c.v.get());
+            precond_apply<T>(c, P, w.p, w.ph);
+            spmv_stored<T>(c, w.ph, w.v);
+            const double gamma = dot_device<T>(c, w.r0, w.v);
             if (!std::isfinite(gamma) || gamma == 0.0) {
                 res.status = CFDG_DIVERGED;
                 break;
             }
             alpha = rho / gamma;
-            cfdg::launch_axpy_copy(N, c.r.get(), -alpha, c.v.get(), c.s.get(), c.stream);
-            precond_apply(c, P, c.s.get(), c.sh.get());
-            spmv(c.sh.get(), c.t.get());
+            cfdg::launch_axpy_copy<T>(N, w.r, -alpha, w.v, w.s, c.stream);
+            precond_apply<T>(c, P, w.s, w.sh);
+            spmv_stored<T>(c, w.sh, w.t);
             {
-                cfdg::DotPairs pr{};
-                const double* A[5] = {c.t.get(), c.t.get(), c.r0.get(), c.r0.get(), c.s.get()};
-                const double* B[5] = {c.t.get(), c.s.get(), c.t.get(), c.s.get(), c.s.get()};
+                cfdg::DotPairsT<T> pr{};
+                const T* A[5] = {w.t, w.t, w.r0, w.r0, w.s};
+                const T* B[5] = {w.t, w.s, w.t, w.s, w.s};
                 for (int j = 0; j < 5; ++j) {
                     pr.a[j] = A[j];
                     pr.b[j] = B[j];
@@ -485,8 +380,7 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                 break;
             }
             const double omega_new = ts / tt;
-            cfdg::launch_update_xr(N, alpha, omega_new, c.ph.get(), c.sh.get(), c.s.get(), c.t.get(), x, c.r.get(),
-                                   c.stream);
+            cfdg::launch_update_xr<T>(N, alpha, omega_new, w.ph, w.sh, w.s, w.t, x, w.r, c.stream);
             const double rn2 = ss - 2.0 * omega_new * ts + omega_new * omega_new * tt;
             rnorm = std::sqrt(std::max(rn2, 0.0));
             const double rho_new = r0s - omega_new * r0t;
@@ -494,7 +388,7 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                 res.status = CFDG_DIVERGED;
                 break;
             }
-            if (it % 32 == 0) rnorm = std::sqrt(dot_device(c, c.r.get(), c.r.get()));
+            if (it % 32 == 0) rnorm = std::sqrt(dot_device<T>(c, w.r, w.r));
             if (rnorm <= tol) {
                 res.status = CFDG_CONVERGED;
                 break;
@@ -515,6 +409,68 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
     return res;
 }
 
+// FP64 defect correction around FP32 BiCGSTAB + SGS (the FP32 precision form):
+// x (FP64, initial guess on entry), b (FP64).  Every pass forms r = b - A x in
+// FP64 arithmetic over the stored blocks and solves A e = r in FP32 from e = 0.
+cfdg_iter_result defect_correction(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& P, double* x,
+                                   const double* bvec, bool zero_fallback = false) {
+    const long long N = 5LL * c.nc;
+    cfdg_iter_result res{CFDG_MAX_ITERATIONS, 0, 0.0, 0.0};
+    const auto outer_residual = [&]() {
+        spmv_stored<double>(c, x, c.v.get());
+        cfdg::launch_axpy_copy<double>(N, bvec, -1.0, c.v.get(), c.r.get(), c.stream);
+        return std::sqrt(dot_device<double>(c, c.r.get(), c.r.get()));
+    };
+    const double bnorm = std::sqrt(dot_device<double>(c, bvec, bvec));
+    double rnorm = outer_residual();
+    if (zero_fallback) {
+        if (!(rnorm <= bnorm)) {
+            c.x.zero(c.stream);
+            rnorm = outer_residual();
+            ++c.warm_rejected;
+        } else {
+            ++c.warm_used;
+        }
+    }
+    const double tol = std::max(lp.abs_tol, lp.rel_tol * ((bnorm > 0.0) ? bnorm : 1.0));
+    res.initial_residual = rnorm;
+    const KVecs<float> w = kvecs32(c);
+    int total = 0;
+    for (int pass = 0;; ++pass) {
+        if (!std::isfinite(rnorm)) {
+            res.status = CFDG_DIVERGED;
+            break;
+        }
+        if (rnorm <= tol) {
+            res.status = CFDG_CONVERGED;
+            break;
+        }
+        if (pass >= c.prec.max_outer || total >= lp.max_iter) {
+            res.status = CFDG_MAX_ITERATIONS;
+            break;
+        }
+        cfdg::launch_cast_f64_to_f32(N, c.r.get(), c.f32.b.get(), c.stream);
+        c.f32.x.zero(c.stream);
+        cfdg_linparams in = lp;
+        in.rel_tol = c.prec.inner_rel_tol;
+        in.abs_tol = 0.5 * tol;
+        in.max_iter = lp.max_iter - total;
+        const cfdg_iter_result ir = bicgstab<float>(c, in, P, w, c.f32.x.get(), c.f32.b.get());
+        total += ir.iterations;
+        if (ir.iterations == 0 && ir.status != CFDG_CONVERGED) {
+            res.status = CFDG_DIVERGED;
+            break;
+        }
+        cfdg::launch_add_f32(N, c.f32.x.get(), x, c.stream);
+        rnorm = outer_residual();
+    }
+    c.check_launch();
+    c.sync();
+    res.iterations = total;
+    res.final_residual = rnorm;
+    return res;
+}
+
 void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const double* hx0, double* hx,
                   cfdg_iter_result* out) {
     if (lp.preconditioner != CFDG_PRECOND_NONE && lp.preconditioner != CFDG_PRECOND_SGS)
@@ -531,10 +487,13 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
         if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "implicit right-hand side needs a residual");
         cfdg::launch_rhs(c.nc, c.nc, c.res.get(), c.b.get(), c.stream);
     }
+    bool learned = false;
     if (hx0) {
         const auto d = c.vec_to_device(hx0);
         c.h2d(c.x.get(), d.data(), n5);
         c.sync();
+    } else if (!hb && c.warm) {
+        learned = cfdg::gnn_warm_guess(c, c.x.get());
     } else {
         c.x.zero(c.stream);
     }
@@ -543,7 +502,9 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
     P.order = lp.sgs_order;
     hip_check(hipEventRecord(c.ev[5], c.stream), "event");
     precond_setup(c, P);
-    const cfdg_iter_result r = bicgstab(c, lp, P, c.x.get(), c.b.get());
+    const cfdg_iter_result r = (c.prec.krylov == CFDG_PREC_FP32)
+                                   ? defect_correction(c, lp, P, c.x.get(), c.b.get(), learned)
+                                   : bicgstab<double>(c, lp, P, kvecs64(c), c.x.get(), c.b.get(), learned);
     hip_check(hipEventRecord(c.ev[6], c.stream), "event");
     c.sync();
     if (hx) {
@@ -618,31 +579,34 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     c.old2new = tl.old2new;
     c.ntiles = static_cast<int>(tl.tile_off.size()) - 1;
 
-    // face records in device numbering
-    std::vector<i32> owner_dev(z(c.nf)), neigh_dev(z(c.nf));
+    // face planes (mesh face order) and (owner, neighbour or ghost) in device numbering
+    std::vector<i32> owner_dev(z(c.nf));
+    std::vector<double> fph;
+    std::vector<int2> fcells(z(c.nf));
+    c.fld = (static_cast<long long>(c.nf) + 63) / 64 * 64;
     {
         std::vector<double> dx, dy, dz, di;
         cfdg::face_distances(m, dx, dy, dz, di);
-        std::vector<double> fr(z(c.nf) * cfdg::kFRec, 0.0);
+        fph.assign(z(c.fld) * cfdg::kFacePlanes, 0.0);
+        const auto put = [&](int k, int f, double v) { fph[z(k) * z(c.fld) + z(f)] = v; };
         for (int f = 0; f < c.nf; ++f) {
-            double* r = fr.data() + z(f) * cfdg::kFRec;
-            r[cfdg::kFN] = m.face_nx[z(f)];
-            r[cfdg::kFN + 1] = m.face_ny[z(f)];
-            r[cfdg::kFN + 2] = m.face_nz[z(f)];
-            r[cfdg::kFA] = m.face_area[z(f)];
-            r[cfdg::kFC] = m.face_cx[z(f)];
-            r[cfdg::kFC + 1] = m.face_cy[z(f)];
-            r[cfdg::kFC + 2] = m.face_cz[z(f)];
-            r[cfdg::kFD] = dx[z(f)];
-            r[cfdg::kFD + 1] = dy[z(f)];
-            r[cfdg::kFD + 2] = dz[z(f)];
-            r[cfdg::kFDinv] = di[z(f)];
+            put(cfdg::kFN, f, m.face_nx[z(f)]);
+            put(cfdg::kFN + 1, f, m.face_ny[z(f)]);
+            put(cfdg::kFN + 2, f, m.face_nz[z(f)]);
+            put(cfdg::kFA, f, m.face_area[z(f)]);
+            put(cfdg::kFC, f, m.face_cx[z(f)]);
+            put(cfdg::kFC + 1, f, m.face_cy[z(f)]);
+            put(cfdg::kFC + 2, f, m.face_cz[z(f)]);
+            put(cfdg::kFD, f, dx[z(f)]);
+            put(cfdg::kFD + 1, f, dy[z(f)]);
+            put(cfdg::kFD + 2, f, dz[z(f)]);
+            put(cfdg::kFDinv, f, di[z(f)]);
             owner_dev[z(f)] = c.old2new[z(m.face_owner[z(f)])];
-            neigh_dev[z(f)] = (m.face_neigh[z(f)] >= 0) ? c.old2new[z(m.face_neigh[z(f)])] : -1;
-            i32 cells[2] = {owner_dev[z(f)], neigh_dev[z(f)]};
-            std::memcpy(r + cfdg::kFCells, cells, sizeof(cells));
+            const i32 other = (f < c.ninner) ? c.old2new[z(m.face_neigh[z(f)])] : c.nc + (f - c.ninner);
+            fcells[z(f)] = int2{owner_dev[z(f)], other};
         }
-        c.frec.upload(fr, s);
+        c.fp.upload(fph, s);
+        c.fcells.upload(fcells, s);
         c.sync();
     }
     {
@@ -659,6 +623,18 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.tile_off.upload(tl.tile_off, s);
         c.tile_face_off.upload(tf.tile_face_off, s);
         c.tile_faces.upload(tf.faces, s);
+        // the flux kernel's copy of the face planes, in the order of the tiles' face lists
+        const long long nte = static_cast<long long>(tf.faces.size());
+        c.tld = (nte + 63) / 64 * 64;
+        std::vector<double> tph(z(c.tld) * cfdg::kFacePlanes, 0.0);
+        std::vector<int2> tcells(z(nte));
+        for (long long e = 0; e < nte; ++e) {
+            const i32 f = tf.faces[z(e)];
+            for (int k = 0; k < cfdg::kFacePlanes; ++k) tph[z(k) * z(c.tld) + z(e)] = fph[z(k) * z(c.fld) + z(f)];
+            tcells[z(e)] = fcells[z(f)];
+        }
+        c.tf.upload(tph, s);
+        c.tcells.upload(tcells, s);
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
         c.ccf_off.upload(ccf_dev.off, s);
         c.ccf.upload(ccf_dev.idx, s);
@@ -682,18 +658,19 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         }
         c.sync();
     }
-    // cell records: static centroid and volume; everything else zero
+    // cell planes: static centroid and volume; everything else zero
     {
-        std::vector<double> cr(z(c.ntot) * cfdg::kRec, 0.0);
+        c.ld = (static_cast<long long>(c.ntot) + 63) / 64 * 64;
+        std::vector<double> cp(z(c.ld) * cfdg::kCellPlanes, 0.0);
+        const auto put = [&](int k, int r, double v) { cp[z(k) * z(c.ld) + z(r)] = v; };
         for (int r = 0; r < c.nc; ++r) {
             const std::size_t o = z(c.new2old[z(r)]);
-            double* p = cr.data() + z(r) * cfdg::kRec;
-            p[cfdg::kRC] = m.cell_cx[o];
-            p[cfdg::kRC + 1] = m.cell_cy[o];
-            p[cfdg::kRC + 2] = m.cell_cz[o];
-            p[cfdg::kRVol] = m.cell_vol[o];
+            put(cfdg::kRC, r, m.cell_cx[o]);
+            put(cfdg::kRC + 1, r, m.cell_cy[o]);
+            put(cfdg::kRC + 2, r, m.cell_cz[o]);
+            put(cfdg::kRVol, r, m.cell_vol[o]);
         }
-        c.crec.upload(cr, s);
+        c.cs.upload(cp, s);
         c.sync();
     }
     // BSR: the mesh-order pattern (bsr_helpers.hpp:16-59) and its device rows
@@ -730,14 +707,16 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     c.U.alloc(5 * ncs);
     c.res.alloc(5 * ncs);
     c.lam.alloc(ncs);
+    c.bflux.alloc(5 * static_cast<std::size_t>(std::max(c.nbf, 1)));
+    c.bint.alloc(5 * static_cast<std::size_t>(std::max(c.npatch, 1)));
     c.dt.alloc(ncs);
     c.alpha.alloc(ncs);
     c.U.zero(s);
     c.res.zero(s);
     c.lam.zero(s);
-    c.values.alloc(c.h_cols.size() * 25);
+    c.values.alloc(c.h_cols.size() * 25 * sizeof(double));
     c.values.zero(s);
-    c.lu.alloc(ncs * 25);
+    c.lu.alloc(ncs * 25 * sizeof(double));
     c.piv.alloc(ncs * 5);
     c.singular.alloc(1);
     for (DBuf<double>* v : {&c.b, &c.x, &c.r, &c.r0, &c.p, &c.v, &c.s, &c.t, &c.ph, &c.sh}) {
@@ -747,6 +726,107 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     c.partial.alloc(static_cast<std::size_t>(cfdg::kReduceBlocks) * 5);
     c.dots.alloc(8);
     c.sync();
+}
+
+// BackwardEuler::advance (implicit_euler.hpp:34-76): residual, dt, Jacobian,
+// (alpha I + J) du = -R, U <- U + du; the step proceeds whatever the linear
+// solver's status (implicit_euler.hpp:45-47).  out->l2 = norms of R(U^n).
+void implicit_step(cfdg_ctx& c, const cfdg_linparams& lp, cfdg_step_result* out) {
+    eval_residual(c);
+    compute_dt(c);
+    cfdg::gnn_step_features(c);
+    assemble_jacobian(c);
+    cfdg_iter_result lr{};
+    linear_solve(c, lp, nullptr, nullptr, nullptr, &lr);
+    double l2[5];
+    residual_norms(c, l2);
+    cfdg::launch_state_update(c.nc, c.nc, c.x.get(), c.U.get(), c.stream);
+    c.check_launch();
+    c.sync();
+    c.have_residual = false;
+    c.have_dt = false;
+    for (int v = 0; v < 5; ++v) out->l2[v] = l2[v];
+    out->linear = lr;
+}
+
+std::size_t storage_bytes(int storage) {
+    return storage == cfdg::kStoreF64 ? sizeof(double) : (storage == cfdg::kStoreF32 ? sizeof(float) : sizeof(cfdg::bf16));
+}
+
+// Device-order block values as double, whatever the storage.
+std::vector<double> download_values(cfdg_ctx& c) {
+    const std::size_t n = c.blk_dev.size() * 25;
+    std::vector<double> out(n);
+    if (c.storage == cfdg::kStoreF64) {
+        c.d2h(out.data(), reinterpret_cast<const double*>(c.values.get()), n);
+        c.sync();
+    } else if (c.storage == cfdg::kStoreF32) {
+        std::vector<float> h(n);
+        c.d2h(h.data(), reinterpret_cast<const float*>(c.values.get()), n);
+        c.sync();
+        for (std::size_t i = 0; i < n; ++i) out[i] = cfdg::to_f64(h[i]);
+    } else {
+        std::vector<cfdg::bf16> h(n);
+        c.d2h(h.data(), reinterpret_cast<const cfdg::bf16*>(c.values.get()), n);
+        c.sync();
+        for (std::size_t i = 0; i < n; ++i) out[i] = cfdg::to_f64(h[i]);
+    }
+    return out;
+}
+
+void upload_values(cfdg_ctx& c, const std::vector<double>& h) {
+    const std::size_t n = h.size();
+    if (c.storage == cfdg::kStoreF64) {
+        c.h2d(reinterpret_cast<double*>(c.values.get()), h.data(), n);
+        c.sync();
+    } else if (c.storage == cfdg::kStoreF32) {
+        std::vector<float> t(n);
+        for (std::size_t i = 0; i < n; ++i) t[i] = cfdg::cvt<float>(h[i]);
+        c.h2d(reinterpret_cast<float*>(c.values.get()), t.data(), n);
+        c.sync();
+    } else {
+        std::vector<cfdg::bf16> t(n);
+        for (std::size_t i = 0; i < n; ++i) t[i] = cfdg::cvt<cfdg::bf16>(h[i]);
+        c.h2d(reinterpret_cast<cfdg::bf16*>(c.values.get()), t.data(), n);
+        c.sync();
+    }
+}
+
+void set_precision(cfdg_ctx& c, const cfdg_precision& p) {
+    const bool st_ok = p.block_storage == CFDG_PREC_FP64 || p.block_storage == CFDG_PREC_FP32 ||
+                       p.block_storage == CFDG_PREC_BF16;
+    if (!st_ok) throw CfdgError(CFDG_ERR_CONFIG, "unknown block storage");
+    if (p.krylov != CFDG_PREC_FP64 && p.krylov != CFDG_PREC_FP32) throw CfdgError(CFDG_ERR_CONFIG, "unknown Krylov precision");
+    if (p.krylov == CFDG_PREC_FP64 && p.block_storage != CFDG_PREC_FP64)
+        throw CfdgError(CFDG_ERR_CONFIG, "FP64 Krylov needs FP64 block storage");
+    if (p.krylov == CFDG_PREC_FP32 && (p.max_outer < 1 || !(p.inner_rel_tol > 0.0 && p.inner_rel_tol < 1.0)))
+        throw CfdgError(CFDG_ERR_CONFIG, "FP32 Krylov needs max_outer >= 1 and 0 < inner_rel_tol < 1");
+    const int storage = p.block_storage;  // CFDG_PREC_* == kStore*
+    const std::size_t nnzb = c.blk_dev.size();
+    const std::size_t ncs = z(c.nc);
+    if (storage != c.storage || c.values.size() != nnzb * 25 * storage_bytes(storage)) {
+        c.values.alloc(0);
+        c.values.alloc(nnzb * 25 * storage_bytes(storage));
+        c.values.zero(c.stream);
+    }
+    const int lu_f32 = (p.krylov == CFDG_PREC_FP32) ? 1 : 0;
+    c.lu.alloc(0);
+    c.lu.alloc(ncs * 25 * (lu_f32 ? sizeof(float) : sizeof(double)));
+    if (p.krylov == CFDG_PREC_FP32) {
+        auto& f = c.f32;
+        for (DBuf<float>* v : {&f.b, &f.x, &f.r, &f.r0, &f.p, &f.v, &f.s, &f.t, &f.ph, &f.sh}) {
+            if (v->size() != 5 * ncs) v->alloc(5 * ncs);
+            v->zero(c.stream);
+        }
+    } else {
+        auto& f = c.f32;
+        for (DBuf<float>* v : {&f.b, &f.x, &f.r, &f.r0, &f.p, &f.v, &f.s, &f.t, &f.ph, &f.sh}) v->alloc(0);
+    }
+    c.sync();
+    c.storage = storage;
+    c.lu_f32 = lu_f32;
+    c.prec = p;
+    c.prec.reserved = 0;
 }
 
 }  // namespace
@@ -822,6 +902,11 @@ int cfdg_init_uniform(cfdg_ctx* c, double rho, const double vel[3], double p) {
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
+        // the uniform state is the freestream of the learned warm start's normalisation
+        c->ref_rho = rho;
+        c->ref_p = p;
+        for (int k = 0; k < 3; ++k) c->ref_vel[k] = vel[k];
+        c->have_ref = true;
     });
 }
 
@@ -835,10 +920,10 @@ int cfdg_eval_residual(cfdg_ctx* c) {
 int cfdg_get_primitives(cfdg_ctx* c, double* q, int64_t ld) {
     return guarded(c, [&] {
         if (!q || ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_primitives: bad array");
-        const auto h = c->download_records();
+        const auto h = c->download_planes();
         for (int r = 0; r < c->ntot; ++r) {
             const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
-            for (int v = 0; v < 5; ++v) q[v * ld + o] = h[z(r) * cfdg::kRec + z(cfdg::kRQ + v)];
+            for (int v = 0; v < 5; ++v) q[v * ld + o] = h[z(cfdg::kRQ + v) * z(c->ld) + z(r)];
         }
     });
 }
@@ -847,14 +932,14 @@ int cfdg_get_gradients(cfdg_ctx* c, double* grad, int64_t ld, double* phi, int64
     return guarded(c, [&] {
         if (grad && ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld < n_total");
         if (phi && ld_phi < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld_phi < n_cells");
-        const auto h = c->download_records();
+        const auto h = c->download_planes();
+        const auto at = [&](int k, int r) { return h[z(k) * z(c->ld) + z(r)]; };
         for (int r = 0; r < c->ntot; ++r) {
             const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
-            const double* p = h.data() + z(r) * cfdg::kRec;
             if (grad)
-                for (int k = 0; k < 15; ++k) grad[k * ld + o] = p[cfdg::kRG + k];
+                for (int k = 0; k < 15; ++k) grad[k * ld + o] = at(cfdg::kRG + k, r);
             if (phi && r < c->nc)
-                for (int v = 0; v < 5; ++v) phi[v * ld_phi + o] = p[cfdg::kRPhi + v];
+                for (int v = 0; v < 5; ++v) phi[v * ld_phi + o] = at(cfdg::kRPhi + v, r);
         }
     });
 }
@@ -910,9 +995,7 @@ int cfdg_get_bsr_pattern(cfdg_ctx* c, int32_t* row_ptr, int32_t* cols, int32_t* 
 int cfdg_get_bsr_values(cfdg_ctx* c, double* values) {
     return guarded(c, [&] {
         if (!values) throw CfdgError(CFDG_ERR_ARG, "null output");
-        std::vector<double> h(c->values.size());
-        c->d2h(h.data(), c->values.get(), h.size());
-        c->sync();
+        const std::vector<double> h = download_values(*c);
         for (std::size_t k = 0; k < c->blk_dev.size(); ++k)
             std::memcpy(values + k * 25, h.data() + z(c->blk_dev[k]) * 25, 25 * sizeof(double));
     });
@@ -921,11 +1004,24 @@ int cfdg_get_bsr_values(cfdg_ctx* c, double* values) {
 int cfdg_set_bsr_values(cfdg_ctx* c, const double* values) {
     return guarded(c, [&] {
         if (!values) throw CfdgError(CFDG_ERR_ARG, "null input");
-        std::vector<double> h(c->values.size());
+        std::vector<double> h(c->blk_dev.size() * 25);
         for (std::size_t k = 0; k < c->blk_dev.size(); ++k)
             std::memcpy(h.data() + z(c->blk_dev[k]) * 25, values + k * 25, 25 * sizeof(double));
-        c->h2d(c->values.get(), h.data(), h.size());
-        c->sync();
+        upload_values(*c, h);
+    });
+}
+
+int cfdg_set_precision(cfdg_ctx* c, const cfdg_precision* prec) {
+    return guarded(c, [&] {
+        if (!prec) throw CfdgError(CFDG_ERR_ARG, "null precision");
+        set_precision(*c, *prec);
+    });
+}
+
+int cfdg_get_precision(cfdg_ctx* c, cfdg_precision* prec) {
+    return guarded(c, [&] {
+        if (!prec) throw CfdgError(CFDG_ERR_ARG, "null output");
+        *prec = c->prec;
     });
 }
 
@@ -949,7 +1045,13 @@ int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* zout
         precond_setup(*c, P);
         const auto d = c->vec_to_device(r);
         c->h2d(c->s.get(), d.data(), n5);
-        precond_apply(*c, P, c->s.get(), c->sh.get());
+        if (c->prec.krylov == CFDG_PREC_FP32) {
+            cfdg::launch_cast_f64_to_f32(static_cast<long long>(n5), c->s.get(), c->f32.s.get(), c->stream);
+            precond_apply<float>(*c, P, c->f32.s.get(), c->f32.sh.get());
+            cfdg::launch_cast_f32_to_f64(static_cast<long long>(n5), c->f32.sh.get(), c->sh.get(), c->stream);
+        } else {
+            precond_apply<double>(*c, P, c->s.get(), c->sh.get());
+        }
         std::vector<double> h(n5);
         c->d2h(h.data(), c->sh.get(), n5);
         c->check_launch();
@@ -964,7 +1066,7 @@ int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
         const std::size_t n5 = 5ULL * z(c->nc);
         const auto d = c->vec_to_device(x);
         c->h2d(c->s.get(), d.data(), n5);
-        cfdg::launch_spmv(c->bargs(), c->values.get(), c->s.get(), c->t.get(), c->stream);
+        spmv_stored<double>(*c, c->s.get(), c->t.get());
         std::vector<double> h(n5);
         c->d2h(h.data(), c->t.get(), n5);
         c->check_launch();
@@ -976,22 +1078,67 @@ int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
 int cfdg_implicit_step(cfdg_ctx* c, const cfdg_linparams* lp, cfdg_step_result* out) {
     return guarded(c, [&] {
         if (!lp) throw CfdgError(CFDG_ERR_ARG, "null linear parameters");
-        eval_residual(*c);
-        compute_dt(*c);
-        assemble_jacobian(*c);
-        cfdg_iter_result lr{};
-        linear_solve(*c, *lp, nullptr, nullptr, nullptr, &lr);
-        double l2[5];
-        residual_norms(*c, l2);
-        cfdg::launch_state_update(c->nc, c->nc, c->x.get(), c->U.get(), c->stream);
+        cfdg_step_result st{};
+        implicit_step(*c, *lp, &st);
+        if (out) *out = st;
+    });
+}
+
+int cfdg_boundary_integrals(cfdg_ctx* c, double* mass, double* energy, double* force, int32_t n_patches) {
+    return guarded(c, [&] {
+        if (!c->have_residual) throw CfdgError(CFDG_ERR_STATE, "boundary_integrals needs an evaluated residual");
+        if (n_patches != c->npatch) throw CfdgError(CFDG_ERR_ARG, "n_patches must equal the mesh's patch count");
+        cfdg::launch_boundary_integrals(c->margs(), c->fargs(), c->npatch, c->bint.get(), c->stream);
         c->check_launch();
+        std::vector<double> h(5 * static_cast<std::size_t>(c->npatch));
+        c->d2h(h.data(), c->bint.get(), h.size());
         c->sync();
-        c->have_residual = false;
-        c->have_dt = false;
-        if (out) {
-            for (int v = 0; v < 5; ++v) out->l2[v] = l2[v];
-            out->linear = lr;
+        for (int p = 0; p < c->npatch; ++p) {
+            const double* o = h.data() + 5 * p;
+            if (mass) mass[p] = o[0];
+            if (energy) energy[p] = o[4];
+            if (force)
+                for (int k = 0; k < 3; ++k) force[3 * p + k] = o[1 + k];
         }
+    });
+}
+
+int cfdg_run_steady(cfdg_ctx* c, const cfdg_linparams* lp, const cfdg_steady_params* sp, cfdg_steady_result* out) {
+    return guarded(c, [&] {
+        if (!lp || !sp || !out) throw CfdgError(CFDG_ERR_ARG, "null argument");
+        if (sp->max_steps < 1 || sp->check_interval < 1 || !(sp->rel_tol >= 0.0))
+            throw CfdgError(CFDG_ERR_ARG, "max_steps >= 1, check_interval >= 1 and rel_tol >= 0 required");
+        *out = cfdg_steady_result{};
+        out->status = CFDG_MAX_ITERATIONS;
+        const auto t0 = std::chrono::steady_clock::now();
+        bool have0 = false;
+        for (int64_t it = 1; it <= sp->max_steps; ++it) {
+            cfdg_step_result st{};
+            implicit_step(*c, *lp, &st);
+            out->steps = it;
+            out->linear_iterations += st.linear.iterations;
+            const bool check = it == 1 || it % sp->check_interval == 0 || it == sp->max_steps;
+            if (!check) continue;
+            if (!std::isfinite(st.l2[0])) {
+                out->status = CFDG_DIVERGED;
+                break;
+            }
+            for (int v = 0; v < 5; ++v) out->l2_last[v] = st.l2[v];
+            if (!have0) {
+                for (int v = 0; v < 5; ++v) out->l2_first[v] = st.l2[v];
+                have0 = true;
+                out->rel = 1.0;
+                continue;
+            }
+            double rel = 0.0;
+            for (int v = 0; v < 5; ++v) rel = std::max(rel, st.l2[v] / std::max(out->l2_first[v], 1.0e-300));
+            out->rel = rel;
+            if (rel <= sp->rel_tol) {
+                out->status = CFDG_CONVERGED;
+                break;
+            }
+        }
+        out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     });
 }
 

@@ -12,6 +12,7 @@
 #include <cstdint>
 
 #include "device_physics.hpp"
+#include "precision.hpp"
 
 namespace cfdg {
 
@@ -20,29 +21,61 @@ constexpr int kTileCells = 256;       // cells per tile (= threads per flux work
 constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux kernel
 constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kGradSlots = 640;       // LDS slots (tile cells + stencil halo) of the staged gradient kernel
+// storage of the Jacobian blocks (values of the precision enumeration of cfdg.h)
+constexpr int kStoreF64 = 0, kStoreF32 = 1, kStoreBF16 = 2;
 
-// Cell record, 32 doubles (256 B, two 128-B lines), one per cell and per
-// boundary ghost: line 0 = what neighbour gathers need (primitives, centroid,
-// limiter, volume), line 1 = the 15 gradient components (3 v + d).
-constexpr int kRec = 32;
+// Cell data: structure of arrays in the device (tile) order, kCellPlanes
+// planes of FieldArgs::ld doubles (ld >= cells + boundary ghosts, a multiple of
+// 64): plane k holds quantity k of every cell and ghost.  Consecutive lanes of
+// a wave work on neighbouring cells of one tile, so every plane access of a
+// wave falls in a few cache lines.
+constexpr int kCellPlanes = 29;
 constexpr int kRQ = 0;     // p, u, v, w, T
 constexpr int kRC = 5;     // centroid x, y, z
 constexpr int kRPhi = 8;   // limiter of the 5 primitives
 constexpr int kRVol = 13;  // volume
-constexpr int kRG = 16;    // gradients, (3 v + d)
+constexpr int kRG = 14;    // gradients, (3 v + d)
 
-// Face record, 16 doubles (128 B, one line): unit normal, area, centroid,
-// owner->neighbour centroid vector (mirrored 2 (x_f - x_c) at boundary faces),
-// its inverse length, then owner and neighbour as two int32 in one slot.
-constexpr int kFRec = 16;
-constexpr int kFN = 0, kFA = 3, kFC = 4, kFD = 7, kFDinv = 10, kFCells = 11;
+// Face data: kFacePlanes planes of MeshArgs::fld doubles in mesh face order
+// (unit normal, area, centroid, owner->neighbour centroid vector -- mirrored
+// 2 (x_f - x_c) at boundary faces -- and its inverse length), plus
+// (owner, neighbour) device ids per face, the neighbour of a boundary face
+// being its ghost n_cells + (f - n_inner).  The flux kernel reads a second
+// copy of the same planes laid out in the order of the tiles' face lists.
+constexpr int kFacePlanes = 11;
+constexpr int kFN = 0, kFA = 3, kFC = 4, kFD = 7, kFDinv = 10;
+
+// Read / write views of one cell or face in the planes.
+struct CellRef {
+    const double* b;
+    long long ld;
+    int c;
+    __host__ __device__ __forceinline__ double operator[](int k) const { return b[k * ld + c]; }
+};
+struct CellRefW {
+    double* b;
+    long long ld;
+    int c;
+    __host__ __device__ __forceinline__ double& operator[](int k) const { return b[k * ld + c]; }
+};
+struct FaceRef {
+    const double* b;
+    long long ld;
+    int f;
+    __host__ __device__ __forceinline__ double operator[](int k) const { return b[k * ld + f]; }
+};
 
 struct MeshArgs {
     int nc;        // cells
     int nf;        // faces
     int ninner;    // faces with a neighbour cell
     int ntiles;
-    const double* frec;
+    const double* fp;         // kFacePlanes x fld, mesh face order
+    long long fld;
+    const int2* fcells;       // (owner, neighbour or ghost) per face, device ids
+    const double* tf;         // kFacePlanes x tld, tile face-list order
+    long long tld;
+    const int2* tcells;       // (owner, neighbour or ghost) per tile face-list entry
     const int* cf_off;        // device cell -> faces, ascending face index
     const int* cf;
     const int* cf_slot;       // position of the face in the tile's face list (negative: neighbour side)
@@ -63,9 +96,11 @@ struct MeshArgs {
 
 struct FieldArgs {
     const double* U;  // conservative, 5 planes of nc (device order)
-    double* crec;     // cell records, (nc + boundary faces) x kRec
+    double* cs;       // cell planes, kCellPlanes x ld
+    long long ld;
     double* res;      // 5 planes of nc
     double* lam;      // nc
+    double* bflux;    // F (inviscid + viscous) of every boundary face, 5 per face
 };
 
 struct NumArgs {
@@ -104,9 +139,12 @@ void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,
                             hipStream_t s);
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
+// out[5 p + v] = sum of F[v] over the boundary faces of patch p, p < n_patches
+void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patches, double* out, hipStream_t s);
 void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
                hipStream_t s);
+// values: nnzb x 25 entries of the storage type (kStoreF64 / kStoreF32 / kStoreBF16)
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
-                     const double* alpha, double* values, hipStream_t s);
+                     const double* alpha, void* values, int storage, hipStream_t s);
 
 }  // namespace cfdg

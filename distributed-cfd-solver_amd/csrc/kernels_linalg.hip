@@ -9,9 +9,16 @@
 //   reductions    dot products: per-block partial sums in a fixed order, then
 //                 one block adds the partials in a fixed order -> the result
 //                 is identical from run to run (no atomics)
+//
+// Precision forms: the kernels are templated on the stored block type TS
+// (double, float, bf16), the diagonal-LU type TL (double, float) and the
+// vector / arithmetic type TV (double, float).  Every combination that uses
+// double for all three performs the FP64 arithmetic of the reference in its
+// order; dot products always accumulate in double.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 
 #include "kernels.hpp"
 #include "kernels_linalg.hpp"
@@ -24,16 +31,19 @@ constexpr int kBlock = 256;
 
 inline dim3 grid_for(long long n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
 
-__global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, const double* values, double* lu,
+[[noreturn]] void throw_bad_combination(const char* what) { throw std::invalid_argument(what); }
+
+template <class TS, class TL>
+__global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, const TS* values, TL* lu,
                                                       std::uint8_t* piv, int* singular) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // fully unrolled so that the 25 entries stay in registers: the row swap is
     // a select over the (compile-time) candidate rows
     double a[25];
-    const double* src = values + static_cast<long long>(diag[i]) * 25;
+    const TS* src = values + static_cast<long long>(diag[i]) * 25;
 #pragma unroll
-    for (int k = 0; k < 25; ++k) a[k] = src[k];
+    for (int k = 0; k < 25; ++k) a[k] = to_f64(src[k]);
     int p[5] = {0, 1, 2, 3, 4};
     bool ok = true;
 #pragma unroll
@@ -73,122 +83,140 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
         atomicMax(singular, i + 1);  // integer flag: largest singular row + 1
         return;
     }
-    double* dst = lu + static_cast<long long>(i) * 25;
+    TL* dst = lu + static_cast<long long>(i) * 25;
 #pragma unroll
-    for (int k = 0; k < 25; ++k) dst[k] = a[k];
+    for (int k = 0; k < 25; ++k) dst[k] = cvt<TL>(a[k]);
 #pragma unroll
     for (int k = 0; k < 5; ++k) piv[static_cast<long long>(i) * 5 + k] = static_cast<std::uint8_t>(p[k]);
 }
 
 // y = A x with five lanes per block row: lane ii forms
 // acc_ii = sum_k (sum_jj a_k[ii][jj] x_col(k)[jj]) in column order.
-__global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, const int* cols, const double* values,
-                                                 const double* x, double* y) {
+template <class TS, class TV>
+__global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, const int* cols, const TS* values,
+                                                 const TV* x, TV* y) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long row = t / 5;
     const int ii = static_cast<int>(t - row * 5);
     if (row >= n) return;
-    double acc = 0.0;
+    TV acc = 0;
     const int kb = row_ptr[row], ke = row_ptr[row + 1];
     for (int k = kb; k < ke; ++k) {
-        const double* a = values + static_cast<long long>(k) * 25 + ii * 5;
-        const double* xb = x + static_cast<long long>(cols[k]) * 5;
-        double s = 0.0;
-        s += a[0] * xb[0];
-        s += a[1] * xb[1];
-        s += a[2] * xb[2];
-        s += a[3] * xb[3];
-        s += a[4] * xb[4];
+        const TS* a = values + static_cast<long long>(k) * 25 + ii * 5;
+        const TV* xb = x + static_cast<long long>(cols[k]) * 5;
+        TV s = 0;
+        s += load_as<TV>(a[0]) * xb[0];
+        s += load_as<TV>(a[1]) * xb[1];
+        s += load_as<TV>(a[2]) * xb[2];
+        s += load_as<TV>(a[3]) * xb[3];
+        s += load_as<TV>(a[4]) * xb[4];
         acc += s;
     }
     y[row * 5 + ii] = acc;
 }
 
 // One Gauss-Seidel update for each row of a group: z_i = LU_i^-1 (r_i - sum_{k != d} A_ik z_k).
+template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
-                                                      const int* diag, const double* values, const double* lu,
-                                                      const std::uint8_t* piv, const double* r, double* z) {
+                                                      const int* diag, const TS* values, const TL* lu,
+                                                      const std::uint8_t* piv, const TV* r, TV* z) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int i = rows[t];
     const int d = diag[i];
-    double s[5];
+    TV s[5];
     for (int ii = 0; ii < 5; ++ii) s[ii] = r[static_cast<long long>(i) * 5 + ii];
     const int kb = row_ptr[i], ke = row_ptr[i + 1];
     for (int k = kb; k < ke; ++k) {
         if (k == d) continue;
-        const double* a = values + static_cast<long long>(k) * 25;
-        const double* zb = z + static_cast<long long>(cols[k]) * 5;
-        const double z0 = zb[0], z1 = zb[1], z2 = zb[2], z3 = zb[3], z4 = zb[4];
+        const TS* a = values + static_cast<long long>(k) * 25;
+        const TV* zb = z + static_cast<long long>(cols[k]) * 5;
+        const TV z0 = zb[0], z1 = zb[1], z2 = zb[2], z3 = zb[3], z4 = zb[4];
         for (int ii = 0; ii < 5; ++ii) {
-            double tt = 0.0;
-            tt += a[ii * 5 + 0] * z0;
-            tt += a[ii * 5 + 1] * z1;
-            tt += a[ii * 5 + 2] * z2;
-            tt += a[ii * 5 + 3] * z3;
-            tt += a[ii * 5 + 4] * z4;
+            TV tt = 0;
+            tt += load_as<TV>(a[ii * 5 + 0]) * z0;
+            tt += load_as<TV>(a[ii * 5 + 1]) * z1;
+            tt += load_as<TV>(a[ii * 5 + 2]) * z2;
+            tt += load_as<TV>(a[ii * 5 + 3]) * z3;
+            tt += load_as<TV>(a[ii * 5 + 4]) * z4;
             s[ii] -= tt;
         }
     }
     // lu_solve_block<5> (bsr_helpers.hpp:325-353)
-    const double* L = lu + static_cast<long long>(i) * 25;
+    const TL* L = lu + static_cast<long long>(i) * 25;
     const std::uint8_t* p = piv + static_cast<long long>(i) * 5;
     for (int k = 0; k < 5; ++k) {
         const int pk = p[k];
         if (pk != k) {
-            double tmp = s[k];
+            TV tmp = s[k];
             // s[pk] with a compile-time-friendly select
-            double sp = s[0];
+            TV sp = s[0];
             for (int q = 1; q < 5; ++q) sp = (q == pk) ? s[q] : sp;
             s[k] = sp;
             for (int q = 0; q < 5; ++q) s[q] = (q == pk) ? tmp : s[q];
         }
     }
     for (int a = 1; a < 5; ++a) {
-        double acc = s[a];
-        for (int b = 0; b < a; ++b) acc -= L[a * 5 + b] * s[b];
+        TV acc = s[a];
+        for (int b = 0; b < a; ++b) acc -= load_as<TV>(L[a * 5 + b]) * s[b];
         s[a] = acc;
     }
     for (int a = 4; a >= 0; --a) {
-        double acc = s[a];
-        for (int b = a + 1; b < 5; ++b) acc -= L[a * 5 + b] * s[b];
-        s[a] = acc / L[a * 5 + a];
+        TV acc = s[a];
+        for (int b = a + 1; b < 5; ++b) acc -= load_as<TV>(L[a * 5 + b]) * s[b];
+        s[a] = acc / load_as<TV>(L[a * 5 + a]);
     }
     for (int ii = 0; ii < 5; ++ii) z[static_cast<long long>(i) * 5 + ii] = s[ii];
 }
 
 // p = ((p * beta) + (-beta * omega) v) + r           (bicgstab.cpp:104-109)
-__global__ void __launch_bounds__(kBlock) k_update_p(long long n, double beta, double nbo, const double* r,
-                                                     const double* v, double* p) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update_p(long long n, double beta, double nbo, const T* r, const T* v,
+                                                     T* p) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double t = p[i] * beta;
-    t += nbo * v[i];
-    t += 1.0 * r[i];
+    T t = p[i] * static_cast<T>(beta);
+    t += static_cast<T>(nbo) * v[i];
+    t += static_cast<T>(1.0) * r[i];
     p[i] = t;
 }
 
 // y = x + a z
-__global__ void __launch_bounds__(kBlock) k_axpy_copy(long long n, const double* x, double a, const double* z,
-                                                      double* y) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_axpy_copy(long long n, const T* x, double a, const T* z, T* y) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    y[i] = x[i] + a * z[i];
+    y[i] = x[i] + static_cast<T>(a) * z[i];
 }
 
 // x = (x + alpha ph) + omega sh;  r = s + (-omega) t   (bicgstab.cpp:145-149)
-__global__ void __launch_bounds__(kBlock) k_update_xr(long long n, double alpha, double omega, const double* ph,
-                                                      const double* sh, const double* s, const double* t, double* x,
-                                                      double* r) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update_xr(long long n, double alpha, double omega, const T* ph,
+                                                      const T* sh, const T* s, const T* t, T* x, T* r) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double xv = x[i];
-    xv += alpha * ph[i];
-    xv += omega * sh[i];
+    T xv = x[i];
+    xv += static_cast<T>(alpha) * ph[i];
+    xv += static_cast<T>(omega) * sh[i];
     x[i] = xv;
-    double rv = s[i];
-    rv += (-omega) * t[i];
+    T rv = s[i];
+    rv += static_cast<T>(-omega) * t[i];
     r[i] = rv;
+}
+
+// dst = src converted (defect correction: FP64 residual -> FP32 inner right-hand side)
+template <class TD, class TS>
+__global__ void __launch_bounds__(kBlock) k_cast(long long n, const TS* src, TD* dst) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = static_cast<TD>(src[i]);
+}
+
+// x += e with e in FP32 (defect correction update, in FP64)
+__global__ void __launch_bounds__(kBlock) k_add_f32(long long n, const float* e, double* x) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + static_cast<double>(e[i]);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -212,13 +240,14 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 }
 
 // Up to five dot products of flat vectors in one pass; partial[b * 5 + j].
-__global__ void __launch_bounds__(kBlock) k_dots(long long n, int ndot, DotPairs pr, double* partial) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_dots(long long n, int ndot, DotPairsT<T> pr, double* partial) {
     __shared__ double lds[kBlock / 64];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         for (int j = 0; j < 5; ++j)
-            if (j < ndot) acc[j] += pr.a[j][i] * pr.b[j][i];
+            if (j < ndot) acc[j] += static_cast<double>(pr.a[j][i]) * static_cast<double>(pr.b[j][i]);
     }
     for (int j = 0; j < 5; ++j) {
         if (j >= ndot) break;
@@ -276,35 +305,104 @@ __global__ void __launch_bounds__(kBlock) k_state_update(int nc, int ld, const d
 
 }  // namespace
 
-void launch_lu_factor(int n, const int* diag, const double* values, double* lu, std::uint8_t* piv, int* singular,
-                      hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_lu_factor, grid_for(n), dim3(kBlock), 0, s, n, diag, values, lu, piv, singular);
+void launch_lu_factor(int n, const int* diag, const void* values, int storage, void* lu, int lu_f32,
+                      std::uint8_t* piv, int* singular, hipStream_t s) {
+    if (n <= 0) return;
+    const dim3 g = grid_for(n), b(kBlock);
+    if (!lu_f32) {
+        if (storage != kStoreF64) throw_bad_combination("FP64 LU needs FP64 blocks");
+        hipLaunchKernelGGL((k_lu_factor<double, double>), g, b, 0, s, n, diag, static_cast<const double*>(values),
+                           static_cast<double*>(lu), piv, singular);
+    } else if (storage == kStoreF32) {
+        hipLaunchKernelGGL((k_lu_factor<float, float>), g, b, 0, s, n, diag, static_cast<const float*>(values),
+                           static_cast<float*>(lu), piv, singular);
+    } else if (storage == kStoreBF16) {
+        hipLaunchKernelGGL((k_lu_factor<bf16, float>), g, b, 0, s, n, diag, static_cast<const bf16*>(values),
+                           static_cast<float*>(lu), piv, singular);
+    } else {
+        hipLaunchKernelGGL((k_lu_factor<double, float>), g, b, 0, s, n, diag, static_cast<const double*>(values),
+                           static_cast<float*>(lu), piv, singular);
+    }
 }
 
-void launch_spmv(const BsrArgs& b, const double* values, const double* x, double* y, hipStream_t s) {
-    if (b.n > 0)
-        hipLaunchKernelGGL(k_spmv, grid_for(5LL * b.n), dim3(kBlock), 0, s, b.n, b.row_ptr, b.cols, values, x, y);
+template <class TV>
+void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s) {
+    if (b.n <= 0) return;
+    const dim3 g = grid_for(5LL * b.n), blk(kBlock);
+    if (storage == kStoreF32)
+        hipLaunchKernelGGL((k_spmv<float, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols,
+                           static_cast<const float*>(values), x, y);
+    else if (storage == kStoreBF16)
+        hipLaunchKernelGGL((k_spmv<bf16, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols, static_cast<const bf16*>(values),
+                           x, y);
+    else
+        hipLaunchKernelGGL((k_spmv<double, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols,
+                           static_cast<const double*>(values), x, y);
+}
+template void launch_spmv<double>(const BsrArgs&, const void*, int, const double*, double*, hipStream_t);
+template void launch_spmv<float>(const BsrArgs&, const void*, int, const float*, float*, hipStream_t);
+
+template <class TV>
+void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
+                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s) {
+    if (count <= 0) return;
+    const dim3 g = grid_for(count), blk(kBlock);
+    if constexpr (sizeof(TV) == sizeof(double)) {
+        if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
+        hipLaunchKernelGGL((k_sgs_group<double, double, double>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
+                           static_cast<const double*>(values), static_cast<const double*>(lu), piv, r, z);
+    } else {
+        if (storage == kStoreF32)
+            hipLaunchKernelGGL((k_sgs_group<float, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
+                               b.diag, static_cast<const float*>(values), static_cast<const float*>(lu), piv, r, z);
+        else if (storage == kStoreBF16)
+            hipLaunchKernelGGL((k_sgs_group<bf16, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
+                               static_cast<const bf16*>(values), static_cast<const float*>(lu), piv, r, z);
+        else
+            hipLaunchKernelGGL((k_sgs_group<double, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
+                               b.diag, static_cast<const double*>(values), static_cast<const float*>(lu), piv, r, z);
+    }
+}
+template void launch_sgs_group<double>(const int*, int, const BsrArgs&, const void*, int, const void*,
+                                       const std::uint8_t*, const double*, double*, hipStream_t);
+template void launch_sgs_group<float>(const int*, int, const BsrArgs&, const void*, int, const void*,
+                                      const std::uint8_t*, const float*, float*, hipStream_t);
+
+template <class T>
+void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_update_p<T>, grid_for(n), dim3(kBlock), 0, s, n, beta, nbo, r, v, p);
+}
+template void launch_update_p<double>(long long, double, double, const double*, const double*, double*, hipStream_t);
+template void launch_update_p<float>(long long, double, double, const float*, const float*, float*, hipStream_t);
+
+template <class T>
+void launch_axpy_copy(long long n, const T* x, double a, const T* z, T* y, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy_copy<T>, grid_for(n), dim3(kBlock), 0, s, n, x, a, z, y);
+}
+template void launch_axpy_copy<double>(long long, const double*, double, const double*, double*, hipStream_t);
+template void launch_axpy_copy<float>(long long, const float*, double, const float*, float*, hipStream_t);
+
+template <class T>
+void launch_update_xr(long long n, double alpha, double omega, const T* ph, const T* sh, const T* sv, const T* t, T* x,
+                      T* r, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_update_xr<T>, grid_for(n), dim3(kBlock), 0, s, n, alpha, omega, ph, sh, sv, t, x, r);
+}
+template void launch_update_xr<double>(long long, double, double, const double*, const double*, const double*,
+                                       const double*, double*, double*, hipStream_t);
+template void launch_update_xr<float>(long long, double, double, const float*, const float*, const float*,
+                                      const float*, float*, float*, hipStream_t);
+
+void launch_cast_f64_to_f32(long long n, const double* src, float* dst, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_cast<float, double>), grid_for(n), dim3(kBlock), 0, s, n, src, dst);
 }
 
-void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const double* values, const double* lu,
-                      const std::uint8_t* piv, const double* r, double* z, hipStream_t s) {
-    if (count > 0)
-        hipLaunchKernelGGL(k_sgs_group, grid_for(count), dim3(kBlock), 0, s, rows, count, b.row_ptr, b.cols, b.diag,
-                           values, lu, piv, r, z);
+void launch_cast_f32_to_f64(long long n, const float* src, double* dst, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL((k_cast<double, float>), grid_for(n), dim3(kBlock), 0, s, n, src, dst);
 }
 
-void launch_update_p(long long n, double beta, double nbo, const double* r, const double* v, double* p,
-                     hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_update_p, grid_for(n), dim3(kBlock), 0, s, n, beta, nbo, r, v, p);
-}
-
-void launch_axpy_copy(long long n, const double* x, double a, const double* z, double* y, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_axpy_copy, grid_for(n), dim3(kBlock), 0, s, n, x, a, z, y);
-}
-
-void launch_update_xr(long long n, double alpha, double omega, const double* ph, const double* sh, const double* sv,
-                      const double* t, double* x, double* r, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_update_xr, grid_for(n), dim3(kBlock), 0, s, n, alpha, omega, ph, sh, sv, t, x, r);
+void launch_add_f32(long long n, const float* e, double* x, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_add_f32, grid_for(n), dim3(kBlock), 0, s, n, e, x);
 }
 
 int reduction_blocks(long long n) {
@@ -314,11 +412,15 @@ int reduction_blocks(long long n) {
     return static_cast<int>(b);
 }
 
-void launch_dots(long long n, int ndot, const DotPairs& pr, double* partial, double* out, hipStream_t s) {
+template <class T>
+void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s) {
     const int nb = reduction_blocks(n);
-    hipLaunchKernelGGL(k_dots, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial);
+    hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out);
 }
+
+template void launch_dots<double>(long long, int, const DotPairsT<double>&, double*, double*, hipStream_t);
+template void launch_dots<float>(long long, int, const DotPairsT<float>&, double*, double*, hipStream_t);
 
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s) {
     const int nb = reduction_blocks(nc);

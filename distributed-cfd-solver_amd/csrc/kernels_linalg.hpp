@@ -1,4 +1,8 @@
 // Launch wrappers of the BSR / Krylov kernels (kernels_linalg.hip).
+//
+// Matrix values are passed untyped with their storage kind (kStoreF64,
+// kStoreF32, kStoreBF16 of kernels.hpp); vectors are typed (double or float)
+// and fix the arithmetic type of the kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,24 +15,37 @@ namespace cfdg {
 
 constexpr int kReduceBlocks = 1024;  // partial sums per reduction (fixed -> deterministic)
 
-struct DotPairs {
-    const double* a[5];
-    const double* b[5];
+template <class T>
+struct DotPairsT {
+    const T* a[5];
+    const T* b[5];
 };
+using DotPairs = DotPairsT<double>;
 
-void launch_lu_factor(int n, const int* diag, const double* values, double* lu, std::uint8_t* piv, int* singular,
-                      hipStream_t s);
-void launch_spmv(const BsrArgs& b, const double* values, const double* x, double* y, hipStream_t s);
-void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const double* values, const double* lu,
-                      const std::uint8_t* piv, const double* r, double* z, hipStream_t s);
-void launch_update_p(long long n, double beta, double nbo, const double* r, const double* v, double* p,
-                     hipStream_t s);
-void launch_axpy_copy(long long n, const double* x, double a, const double* z, double* y, hipStream_t s);
-void launch_update_xr(long long n, double alpha, double omega, const double* ph, const double* sh, const double* sv,
-                      const double* t, double* x, double* r, hipStream_t s);
+// Diagonal-block LU with pivoting, factorised in FP64 from the stored blocks
+// and kept in FP64 (lu_f32 = 0, FP64 blocks only) or FP32 (lu_f32 = 1).
+void launch_lu_factor(int n, const int* diag, const void* values, int storage, void* lu, int lu_f32,
+                      std::uint8_t* piv, int* singular, hipStream_t s);
+template <class TV>
+void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s);
+// TV = double: FP64 blocks and FP64 LU.  TV = float: any storage, FP32 LU.
+template <class TV>
+void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
+                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s);
+template <class T>
+void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s);
+template <class T>
+void launch_axpy_copy(long long n, const T* x, double a, const T* z, T* y, hipStream_t s);
+template <class T>
+void launch_update_xr(long long n, double alpha, double omega, const T* ph, const T* sh, const T* sv, const T* t, T* x,
+                      T* r, hipStream_t s);
+void launch_cast_f64_to_f32(long long n, const double* src, float* dst, hipStream_t s);
+void launch_cast_f32_to_f64(long long n, const float* src, double* dst, hipStream_t s);
+void launch_add_f32(long long n, const float* e, double* x, hipStream_t s);
 int reduction_blocks(long long n);
-// out[0..ndot) = dot products; partial needs kReduceBlocks * 5 doubles
-void launch_dots(long long n, int ndot, const DotPairs& pr, double* partial, double* out, hipStream_t s);
+// out[0..ndot) = dot products accumulated in FP64; partial needs kReduceBlocks * 5 doubles
+template <class T>
+void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s);
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s);
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s);
 void launch_state_update(int nc, int ld, const double* du, double* U, hipStream_t s);

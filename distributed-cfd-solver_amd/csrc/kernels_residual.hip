@@ -1,6 +1,6 @@
 // Residual, time-step and Jacobian kernels (gfx950).
 //
-// Data: cell records (256 B) and face records (128 B), kernels.hpp.  Cells are
+// Data: cell and face planes (structure of arrays, kernels.hpp).  Cells are
 // in the device (tile) order; faces keep their mesh index.
 //
 //   k_primitives        U -> (p,u,v,w,T) into the cell records
@@ -28,10 +28,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ int face_owner(const double* fr) { return reinterpret_cast<const int*>(fr + kFCells)[0]; }
-__device__ __forceinline__ int face_neigh(const double* fr) { return reinterpret_cast<const int*>(fr + kFCells)[1]; }
-__device__ __forceinline__ const double* frec(const MeshArgs& m, int f) { return m.frec + static_cast<long long>(f) * kFRec; }
-__device__ __forceinline__ double* crec(const FieldArgs& f, int c) { return f.crec + static_cast<long long>(c) * kRec; }
+__device__ __forceinline__ int face_owner(const MeshArgs& m, int f) { return m.fcells[f].x; }
+// neighbour cell, or the ghost n_cells + (f - n_inner) of a boundary face
+__device__ __forceinline__ int face_other(const MeshArgs& m, int f) { return m.fcells[f].y; }
+__device__ __forceinline__ FaceRef frec(const MeshArgs& m, int f) { return FaceRef{m.fp, m.fld, f}; }
+__device__ __forceinline__ CellRef crec(const FieldArgs& f, int c) { return CellRef{f.cs, f.ld, c}; }
+__device__ __forceinline__ CellRefW crecw(const FieldArgs& f, int c) { return CellRefW{f.cs, f.ld, c}; }
 
 // U -> q for the cells (solver.hpp:185-188)
 __global__ void __launch_bounds__(kBlock) k_primitives(MeshArgs m, FieldArgs f, NumArgs nm) {
@@ -40,12 +42,13 @@ __global__ void __launch_bounds__(kBlock) k_primitives(MeshArgs m, FieldArgs f, 
     double U[5], q[5];
     for (int v = 0; v < 5; ++v) U[v] = f.U[static_cast<long long>(v) * m.nc + c];
     cons_to_prim(nm.gas, U, q);
-    double* r = crec(f, c);
+    const CellRefW r = crecw(f, c);
     for (int v = 0; v < 5; ++v) r[kRQ + v] = q[v];
 }
 
 // 1 / max((x_f - x_c) . n, 1e-14)   (boundary_traversal.hpp:98-108)
-__device__ __forceinline__ double rcfn_inverse(const double* fr, const double* cr) {
+template <class FR, class CR>
+__device__ __forceinline__ double rcfn_inverse(const FR& fr, const CR& cr) {
     const double rx = fr[kFC] - cr[kRC], ry = fr[kFC + 1] - cr[kRC + 1], rz = fr[kFC + 2] - cr[kRC + 2];
     const double r = rx * fr[kFN] + ry * fr[kFN + 1] + rz * fr[kFN + 2];
     return 1.0 / dmax(r, 1.0e-14);
@@ -57,9 +60,9 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
     const int nbf = m.nf - m.ninner;
     if (bf >= nbf) return;
     const int fc = m.ninner + bf;
-    const double* fr = frec(m, fc);
-    const double* ri = crec(f, face_owner(fr));
-    double* rg = crec(f, m.nc + bf);
+    const FaceRef fr = frec(m, fc);
+    const CellRef ri = crec(f, face_owner(m, fc));
+    const CellRefW rg = crecw(f, m.nc + bf);
     const BcDev bp = bt.bc[m.bface_bc[bf]];
     double qi[5], qg[5];
     for (int v = 0; v < 5; ++v) qi[v] = ri[kRQ + v];
@@ -131,11 +134,11 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
 // :197-278; muscl.hpp:26-104).  `self`, nbn(j) and nbf(j) point at 8 doubles
 // laid out as the first half of a cell record (q, centroid) for the cell, its
 // j-th vertex neighbour and its j-th face neighbour -- either the records in
-// HBM or their copies staged in LDS.
-template <class NbN, class NbF>
+// planes in HBM or their copies staged in LDS.
+template <class Self, class NbN, class NbF>
 __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
-                                                      const double* self, NbN nbn, NbF nbf) {
-    double* rc = crec(f, c);
+                                                      const Self& self, NbN nbn, NbF nbf) {
+    const CellRefW rc = crecw(f, c);
     double qc[5];
     for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
     const double x0 = self[kRC], y0 = self[kRC + 1], z0 = self[kRC + 2];
@@ -152,7 +155,7 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
         if (je > jb) {
             double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
             for (int j = jb; j < je; ++j) {
-                const double* rn = nbn(j);
+                const auto rn = nbn(j);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
@@ -182,7 +185,7 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
                 inv[8] = (A0 * A4 - A1 * A3) * id;
             }
             for (int j = jb; j < je; ++j) {
-                const double* rn = nbn(j);
+                const auto rn = nbn(j);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
@@ -203,14 +206,14 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
         // cell-based variants form these sums)
         for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
             const int fc = m.cf[e];
-            const double* fr = frec(m, fc);
-            const int o = face_owner(fr);
-            const int nb = (fc < m.ninner) ? face_neigh(fr) : m.nc + (fc - m.ninner);
+            const FaceRef fr = frec(m, fc);
+            const int o = face_owner(m, fc);
+            const int nb = face_other(m, fc);
             const double a05 = 0.5 * fr[kFA];
             const double hx = a05 * fr[kFN], hy = a05 * fr[kFN + 1], hz = a05 * fr[kFN + 2];
             const bool own = (o == c);
-            const double* ro = crec(f, o);
-            const double* rn = crec(f, nb);
+            const CellRef ro = crec(f, o);
+            const CellRef rn = crec(f, nb);
             for (int v = 0; v < 5; ++v) {
                 const double s = ro[kRQ + v] + rn[kRQ + v];
                 const double fx = hx * s, fy = hy * s, fz = hz * s;
@@ -239,7 +242,7 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
     double qmax[5], qmin[5];
     for (int v = 0; v < 5; ++v) qmax[v] = qmin[v] = qc[v];
     for (int j = m.ccf_off[c]; j < m.ccf_off[c + 1]; ++j) {
-        const double* rn = nbf(j);
+        const auto rn = nbf(j);
         for (int v = 0; v < 5; ++v) {
             const double qn = rn[kRQ + v];
             qmax[v] = dmax(qmax[v], qn);
@@ -249,7 +252,7 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
     double ph[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
     const double eps2 = nm.k3 * rc[kRVol];
     for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const double* fr = frec(m, m.cf[e]);
+        const FaceRef fr = frec(m, m.cf[e]);
         const double ex = fr[kFC] - x0, ey = fr[kFC + 1] - y0, ez = fr[kFC + 2] - z0;
         for (int v = 0; v < 5; ++v) {
             const double df = g[v][0] * ex + g[v][1] * ey + g[v][2] * ez;
@@ -265,34 +268,35 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
     const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
     gradient_limiter_cell(
-        m, f, nm, c, crec(f, c), [&](int j) { return static_cast<const double*>(crec(f, m.ccn[j])); },
-        [&](int j) { return static_cast<const double*>(crec(f, m.ccf[j])); });
+        m, f, nm, c, crec(f, c), [&](int j) { return crec(f, m.ccn[j]); }, [&](int j) { return crec(f, m.ccf[j]); });
 }
 
-// One workgroup per tile: the (q, centroid) halves of the records of the
-// tile's cells and of its stencil halo are staged once into LDS, then every
-// cell reads its vertex and face neighbours from there.
+// View of one LDS slot of the staged gradient kernel: plane k of slot s at
+// sq[k * kGradSlots + s].
+struct LdsSlot {
+    const double* sq;
+    int s;
+    __device__ __forceinline__ double operator[](int k) const { return sq[k * kGradSlots + s]; }
+};
+
+// One workgroup per tile: the (q, centroid) planes of the tile's cells and of
+// its stencil halo are staged once into LDS (structure of arrays, so that
+// lanes reading different slots of one plane spread over the banks), then
+// every cell reads its vertex and face neighbours from there.
 __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm) {
-    __shared__ double sq[kGradSlots * 8];
+    __shared__ double sq[8 * kGradSlots];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
     for (int sl = threadIdx.x; sl < ns; sl += kTileCells) {
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
-        const double2* r = reinterpret_cast<const double2*>(crec(f, src));
-        double2* d = reinterpret_cast<double2*>(sq + sl * 8);
-        const double2 a = r[0], b = r[1], cc = r[2], e = r[3];
-        d[0] = a;
-        d[1] = b;
-        d[2] = cc;
-        d[3] = e;
+        for (int k = 0; k < 8; ++k) sq[k * kGradSlots + sl] = f.cs[k * f.ld + src];
     }
     __syncthreads();
     if (static_cast<int>(threadIdx.x) >= nt) return;
     gradient_limiter_cell(
-        m, f, nm, c0 + threadIdx.x, sq + threadIdx.x * 8,
-        [&](int j) { return static_cast<const double*>(sq + m.ccn_slot[j] * 8); },
-        [&](int j) { return static_cast<const double*>(sq + m.ccf_slot[j] * 8); });
+        m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)},
+        [&](int j) { return LdsSlot{sq, m.ccn_slot[j]}; }, [&](int j) { return LdsSlot{sq, m.ccf_slot[j]}; });
 }
 
 __device__ __forceinline__ void grad_dirichlet(double* g, const double* gi, double qin, double qfix, double nx,
@@ -330,9 +334,9 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
     const int nbf = m.nf - m.ninner;
     if (bf >= nbf) return;
     const int fc = m.ninner + bf;
-    const double* fr = frec(m, fc);
-    const double* ri = crec(f, face_owner(fr));
-    double* rg = crec(f, m.nc + bf);
+    const FaceRef fr = frec(m, fc);
+    const CellRef ri = crec(f, face_owner(m, fc));
+    const CellRefW rg = crecw(f, m.nc + bf);
     const BcDev bp = bt.bc[m.bface_bc[bf]];
     const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2];
     const double rinv = rcfn_inverse(fr, ri);
@@ -386,14 +390,12 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
 // Inviscid + viscous flux of one face times its area, and its spectral-radius
 // contribution (residual_kernel.hpp:183-240 / 245-300 for one face, with the
 // MUSCL states of muscl.hpp:108-157).
-__device__ __forceinline__ void face_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int fc,
-                                          double F[5], double& lam_f) {
-    const double* fr = frec(m, fc);
-    const bool inner = fc < m.ninner;
-    const int c0 = face_owner(fr);
-    const int c1 = inner ? face_neigh(fr) : m.nc + (fc - m.ninner);
-    const double* r0 = crec(f, c0);
-    const double* r1 = crec(f, c1);
+// fr views the face's planes, (c0, c1) = (owner, neighbour or ghost).
+template <class FR>
+__device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm, const FR& fr, bool inner, int c0,
+                                          int c1, double F[5], double& lam_f) {
+    const CellRef r0 = crec(f, c0);
+    const CellRef r1 = crec(f, c1);
     const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
     double qL[5], qR[5];
     if (inner && nm.muscl) {
@@ -448,9 +450,13 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
     const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
     for (int i = threadIdx.x; i < nfl; i += kTileCells) {
         double F[5], lam_f;
-        face_flux(m, f, nm, m.tile_faces[fb + i], F, lam_f);
+        const int fc = m.tile_faces[fb + i];
+        const int2 cc = m.tcells[fb + i];
+        face_flux(f, nm, FaceRef{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
         for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
         sF[5][i] = lam_f;
+        if (fc >= m.ninner)  // a boundary face lies in exactly one tile: single writer
+            for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(fc - m.ninner) * 5 + v] = F[v];
     }
     __syncthreads();
     const int c = m.tile_off[t] + threadIdx.x;
@@ -473,11 +479,11 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
 }
 
 // Implicit local time step and diagonal shift (solver.hpp:256-287)
-__global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cr, const double* lam, double cfl, double* dt,
-                                               double* alpha) {
+__global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cs, long long ld, const double* lam, double cfl,
+                                               double* dt, double* alpha) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const double vol = cr[static_cast<long long>(c) * kRec + kRVol];
+    const double vol = cs[kRVol * ld + c];
     const double inv_cfl = 1.0 / cfl;
     const double l = dmax(lam[c], 1.0e-14);
     dt[c] = cfl * vol / l;
@@ -494,21 +500,22 @@ __device__ __forceinline__ int find_col(const BsrArgs& b, int row, int col) {
 // First-order Picard Jacobian, one thread per block row (jacobian_kernel.hpp:
 // 74-270): all blocks of row c come from the thread of row c; the diagonal
 // block accumulates over the cell's faces in ascending face index.
+template <class TS>
 __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
-                                                     const double* alpha, double* values) {
+                                                     const double* alpha, TS* values) {
     const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
     double dg[25];
     for (int i = 0; i < 25; ++i) dg[i] = 0.0;
     for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
         const int fc = m.cf[e];
-        const double* fr = frec(m, fc);
+        const FaceRef fr = frec(m, fc);
         const bool inner = fc < m.ninner;
-        const int c0 = face_owner(fr);
-        const int c1 = inner ? face_neigh(fr) : m.nc + (fc - m.ninner);
+        const int c0 = face_owner(m, fc);
+        const int c1 = face_other(m, fc);
         const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
-        const double* r0 = crec(f, c0);
-        const double* r1 = crec(f, c1);
+        const CellRef r0 = crec(f, c0);
+        const CellRef r1 = crec(f, c1);
         double qL[5], qR[5], UL[5], UR[5];
         for (int v = 0; v < 5; ++v) {
             qL[v] = r0[kRQ + v];
@@ -541,18 +548,43 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, Nu
                     for (int i = 0; i < 25; ++i) dg[i] -= J[i];
             } else {
                 const int k = find_col(b, c, own ? c1 : c0);
-                double* dst = values + static_cast<long long>(k) * 25;
+                TS* dst = values + static_cast<long long>(k) * 25;
                 if (own)
-                    for (int i = 0; i < 25; ++i) dst[i] = 0.0 + J[i];
+                    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(0.0 + J[i]);
                 else
-                    for (int i = 0; i < 25; ++i) dst[i] = 0.0 - J[i];
+                    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(0.0 - J[i]);
             }
         }
     }
     const double a = alpha[c];
     for (int i = 0; i < 5; ++i) dg[i * 6] += a;
-    double* dst = values + static_cast<long long>(b.diag[c]) * 25;
-    for (int i = 0; i < 25; ++i) dst[i] = dg[i];
+    TS* dst = values + static_cast<long long>(b.diag[c]) * 25;
+    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(dg[i]);
+}
+
+// Per-patch boundary integrals (residual_kernel.hpp:96-169 forms F[0] and
+// F[4]; the momentum components give the force on the patch).  One workgroup
+// per patch; thread t sums the patch's faces t, t + 256, ... in order, then a
+// fixed tree adds the thread sums.
+__global__ void __launch_bounds__(kBlock) k_boundary_integrals(int nbf, const int* bface_bc, const double* bflux,
+                                                               double* out) {
+    __shared__ double red[kBlock];
+    const int p = blockIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nbf; b += kBlock) {
+        if (bface_bc[b] != p) continue;
+        for (int v = 0; v < 5; ++v) acc[v] += bflux[static_cast<long long>(b) * 5 + v];
+    }
+    for (int v = 0; v < 5; ++v) {
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[p * 5 + v] = red[0];
+        __syncthreads();
+    }
 }
 
 inline dim3 grid_for(long long n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
@@ -587,14 +619,29 @@ void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipSt
     if (m.ntiles > 0) hipLaunchKernelGGL(k_flux, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
 }
 
+void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patches, double* out, hipStream_t s) {
+    if (n_patches > 0)
+        hipLaunchKernelGGL(k_boundary_integrals, dim3(static_cast<unsigned>(n_patches)), dim3(kBlock), 0, s,
+                           m.nf - m.ninner, m.bface_bc, f.bflux, out);
+}
+
 void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
                hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.crec, lam, cfl, dt, alpha);
+    if (m.nc > 0) hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.cs, f.ld, lam, cfl, dt, alpha);
 }
 
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
-                     const double* alpha, double* values, hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_jacobian, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha, values);
+                     const double* alpha, void* values, int storage, hipStream_t s) {
+    if (m.nc <= 0) return;
+    if (storage == kStoreF32)
+        hipLaunchKernelGGL(k_jacobian<float>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
+                           static_cast<float*>(values));
+    else if (storage == kStoreBF16)
+        hipLaunchKernelGGL(k_jacobian<bf16>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
+                           static_cast<bf16*>(values));
+    else
+        hipLaunchKernelGGL(k_jacobian<double>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
+                           static_cast<double*>(values));
 }
 
 }  // namespace cfdg

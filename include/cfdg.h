@@ -32,6 +32,7 @@
  *   cfdg_get_gradients     gradient planes + limiters (solver.hpp:205-225)
  *   cfdg_get_residual      res slots + spectral radius (residual_kernel.hpp:64-91)
  *   cfdg_residual_norms    Solver::residual_norms (solver.hpp:740-756)
+ *   cfdg_boundary_integrals ResidualKernel::boundary_integrals (residual_kernel.hpp:96-169)
  *   cfdg_compute_dt        Solver::compute_dt (solver.hpp:256-287)
  *   cfdg_assemble_jacobian JacobianKernel::apply (jacobian_kernel.hpp:74-103)
  *   cfdg_get_bsr*          BsrMatrix::{rowPtr, cols, diagIndex, valuesData}
@@ -42,6 +43,7 @@
  *   cfdg_sgs_apply         SgsPreconditioner::{setup, apply} (preconditioners.cpp:62-194)
  *   cfdg_spmv              BsrMatrix::apply (bsr_matrix.cpp:10-20)
  *   cfdg_implicit_step     BackwardEuler::advance (implicit_euler.hpp:34-76)
+ *   cfdg_run_steady        Solver::run_steady (solver.hpp:594-654)
  */
 #ifndef CFDG_H
 #define CFDG_H
@@ -199,6 +201,13 @@ int cfdg_get_gradients(cfdg_ctx* ctx, double* grad, int64_t ld, double* phi, int
                                                                         /* grad[(3 v + d) ld + c] */
 int cfdg_get_residual(cfdg_ctx* ctx, double* R, int64_t ld, double* lam); /* 5 x n_cells, n_cells */
 int cfdg_residual_norms(cfdg_ctx* ctx, double l2[5]);
+/* Per-patch boundary integrals of the last evaluated residual
+   (ResidualKernel::boundary_integrals, residual_kernel.hpp:96-169): mass[p] =
+   net mass flux, energy[p] = inviscid + viscous energy flux through patch p.
+   force[3 p + k] (no reference counterpart) = sum of the momentum fluxes
+   F[1 + k] (inviscid + viscous), i.e. the force of the fluid on the patch.
+   Any output may be NULL; n_patches must equal the mesh's patch count. */
+int cfdg_boundary_integrals(cfdg_ctx* ctx, double* mass, double* energy, double* force, int32_t n_patches);
 int cfdg_compute_dt(cfdg_ctx* ctx);
 int cfdg_get_dt(cfdg_ctx* ctx, double* dt, double* alpha);
 
@@ -243,6 +252,115 @@ typedef struct {
 /* One BackwardEuler::advance: residual, dt, Jacobian, solve (alpha I + J) du = -R,
    U <- U + du. */
 int cfdg_implicit_step(cfdg_ctx* ctx, const cfdg_linparams* lp, cfdg_step_result* out);
+
+/* Steady run: implicit steps until the relative residual
+   max_v L2_v(R(U^n)) / max(L2_v(R(U^0)), 1e-300) <= rel_tol, checked at step 1
+   and every check_interval steps, or max_steps steps (Solver::run_steady,
+   solver.hpp:594-654; residual_norms / relative_residual, :740-765).  A
+   non-finite norm stops the run with status CFDG_DIVERGED (the reference
+   dumps the state and returns 1). */
+typedef struct {
+    int64_t max_steps;
+    double rel_tol;
+    int32_t check_interval;   /* residual_interval, >= 1 */
+} cfdg_steady_params;
+
+typedef struct {
+    int32_t status;             /* CFDG_CONVERGED / CFDG_DIVERGED / CFDG_MAX_ITERATIONS */
+    int64_t steps;
+    int64_t linear_iterations;  /* summed over the steps */
+    double rel;                 /* last relative residual */
+    double l2_first[5], l2_last[5];
+    double seconds;             /* wall time of the run */
+} cfdg_steady_result;
+
+int cfdg_run_steady(cfdg_ctx* ctx, const cfdg_linparams* lp, const cfdg_steady_params* sp,
+                    cfdg_steady_result* out);
+
+/* ---- precision forms ------------------------------------------------------
+   No counterpart in the reference, whose linear algebra is FP64 throughout
+   (BsrMatrix values are double, bsr_matrix.hpp:24-62).  The default form is
+   FP64 blocks + FP64 Krylov, i.e. the reference's arithmetic.  With
+   krylov = FP32 the Jacobian blocks are stored in FP32 or BF16 (the SGS
+   diagonal LU is factorised in FP64 from the stored blocks and kept in FP32),
+   and cfdg_linear_solve becomes an FP64 defect correction: r = b - A x in FP64
+   arithmetic over the stored blocks, an FP32 BiCGSTAB + FP32 SGS solve of
+   A e = r to inner_rel_tol, x += e, repeated until ||r|| meets the caller's
+   max(abs_tol, rel_tol ||b||) or max_iter inner iterations / max_outer passes
+   are spent.  iterations counts inner iterations; final_residual is the FP64
+   ||b - A x||.  Dot products accumulate in FP64 in every form.  The flux
+   residual, gradients, norms and the state update stay FP64. */
+#define CFDG_PREC_FP64 0
+#define CFDG_PREC_FP32 1
+#define CFDG_PREC_BF16 2
+
+typedef struct {
+    int32_t block_storage;  /* CFDG_PREC_FP64 | CFDG_PREC_FP32 | CFDG_PREC_BF16 */
+    int32_t krylov;         /* CFDG_PREC_FP64 (needs FP64 blocks) | CFDG_PREC_FP32 */
+    int32_t max_outer;      /* defect-correction passes (>= 1) */
+    int32_t reserved;       /* 0 */
+    double inner_rel_tol;   /* each inner solve stops at inner_rel_tol ||r|| (0 < tol < 1) */
+} cfdg_precision;
+
+/* Selects the form; reallocates the block storage (the next solve needs a new
+   cfdg_assemble_jacobian or cfdg_set_bsr_values).  cfdg_get_bsr_values /
+   cfdg_set_bsr_values convert between the storage and double; cfdg_spmv
+   multiplies the stored blocks in FP64; cfdg_sgs_apply runs the form's SGS. */
+int cfdg_set_precision(cfdg_ctx* ctx, const cfdg_precision* prec);
+int cfdg_get_precision(cfdg_ctx* ctx, cfdg_precision* prec);
+
+/* ---- learned warm start ---------------------------------------------------
+   No counterpart in the reference (its BiCGSTAB always starts from x = 0,
+   implicit_euler.hpp:81-92; the solver API takes the guess in x,
+   bicgstab.cpp:56-60).  A two-layer edge-convolution network over the face
+   graph (hidden width 64) predicts the update du of each implicit step from
+   per-cell features (state, residual over spectral radius -R / lambda, volume, local time
+   step, freestream Mach) and face geometry.  Node products run in BF16 on
+   MFMA with FP32 accumulation; training (MSE on normalised updates, Adam)
+   uses the library's own forward and backward kernels.  The prediction is
+   the initial guess of the linear solve; when its initial residual
+   ||b - A x0|| exceeds ||b|| the solve restarts from zero (fallback), and the
+   linear tolerance is unchanged.  Parameter layout: see cfdg_gnn_get_params. */
+typedef struct cfdg_gnn cfdg_gnn;
+
+/* Glorot-uniform weights from the seed, zero biases. */
+int cfdg_gnn_create(uint64_t seed, cfdg_gnn** out);
+int cfdg_gnn_destroy(cfdg_gnn* gnn);
+const char* cfdg_gnn_last_error(const cfdg_gnn* gnn);
+int cfdg_gnn_num_params(int32_t* n);
+/* Flat FP32 parameters in the order [WP1; WQ1] (128 x 32, columns >= 16
+   zero), G1 (64 x 4), b1 (64), [WP2; WQ2] (128 x 64), G2 (64 x 4), b2 (64),
+   Wo (5 x 64), bo (5); all row-major.  set_params resets the Adam state. */
+int cfdg_gnn_get_params(cfdg_gnn* gnn, float* params, int32_t n);
+int cfdg_gnn_set_params(cfdg_gnn* gnn, const float* params, int32_t n);
+
+/* Freestream of the feature normalisation (cfdg_init_uniform sets it too). */
+int cfdg_set_freestream(cfdg_ctx* ctx, double rho, const double velocity[3], double p);
+/* on != 0: every implicit step keeps its features; cfdg_gnn_record then adds
+   the last step's (features, du) pair as one training sample (the context's
+   graph is copied once per context). */
+int cfdg_gnn_set_recording(cfdg_ctx* ctx, int32_t on);
+int cfdg_gnn_record(cfdg_gnn* gnn, cfdg_ctx* ctx);
+int cfdg_gnn_num_samples(cfdg_gnn* gnn, int32_t* n);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over all samples, epochs passes in a
+   seeded order, one step per sample; mean_loss = mean loss of the last epoch. */
+int cfdg_gnn_train(cfdg_gnn* gnn, int32_t epochs, float lr, uint64_t seed, double* mean_loss);
+/* Loss and full parameter gradient of one sample, parameters unchanged. */
+int cfdg_gnn_loss_grad(cfdg_gnn* gnn, int32_t sample, double* loss, float* grad, int32_t n);
+/* A recorded sample: sizes (any array may be NULL), features n x 32 (the
+   stored BF16 values), targets n x 5, graph (device order) with per-edge
+   geometry e x 4. */
+int cfdg_gnn_get_sample(cfdg_gnn* gnn, int32_t sample, int32_t* n_nodes, int32_t* n_edges, float* x, float* t,
+                        int32_t* adj_off, int32_t* adj, int32_t* rev, float* eg);
+/* Predicted du (AoS, mesh order) for the current state; needs
+   cfdg_eval_residual and cfdg_compute_dt. */
+int cfdg_gnn_predict(cfdg_gnn* gnn, cfdg_ctx* ctx, double* du);
+/* Initial guess of cfdg_implicit_step / cfdg_linear_solve(b = NULL, x0 =
+   NULL): the network's prediction (gnn != NULL) or zero (gnn == NULL).  The
+   network must outlive its use by the context. */
+int cfdg_set_warm_start(cfdg_ctx* ctx, cfdg_gnn* gnn);
+/* Solves since cfdg_set_warm_start that kept / rejected the prediction. */
+int cfdg_warm_start_stats(cfdg_ctx* ctx, int64_t* used, int64_t* rejected);
 
 /* Device time (milliseconds, HIP events on the context stream) of the last
    call of each stage: 0 primitives+ghosts, 1 gradients, 2 limiter,

@@ -407,6 +407,7 @@ struct oracle_ctx {
     // fields (SoA, n_total)
     std::vector<double> U, q, gx, gy, gz, phi, res, lam, dt, alpha;
     std::vector<double> fscale;  // sum over the cell's faces of |F|, per variable
+    std::vector<double> bflux;   // F (inviscid + viscous) of every boundary face, 5 per face
     bool have_residual = false;
     // BSR
     std::vector<i32> row_ptr, cols, diag;
@@ -1051,6 +1052,8 @@ void compute_fluxes(oracle_ctx& c) {
             lam_f += lv;
         }
         for (int v = 0; v < 5; ++v) plane(c.res, c.ntot, v)[c0] += F[v];
+        if (!inner)
+            for (int v = 0; v < 5; ++v) c.bflux[(fs - static_cast<size_t>(c.ninner)) * 5 + static_cast<size_t>(v)] = F[v];
         c.lam[c0] += lam_f;
         for (int v = 0; v < 5; ++v) plane(c.fscale, c.ntot, v)[c0] += std::fabs(F[v]);
         if (inner) {
@@ -1458,6 +1461,7 @@ int oracle_create(const oracle_mesh_view* m, const oracle_numerics* num, const o
         c->phi.assign(5 * nt, 0.0);
         c->res.assign(5 * nt, 0.0);
         c->fscale.assign(5 * nt, 0.0);
+        c->bflux.assign(5 * static_cast<size_t>(c->nf - c->ninner), 0.0);
         c->lam.assign(nc, 0.0);
         c->dt.assign(nc, 0.0);
         c->alpha.assign(nc, 0.0);
@@ -1693,6 +1697,37 @@ int oracle_sgs_apply(oracle_ctx* c, int32_t sgs_order, const double* r, double* 
     s.order = (sgs_order == 1) ? 1 : 0;
     if (s.order == 0) colour_rows(*c, s.colour, s.n_colours);
     sgs_apply(*c, s, r, z);
+    return 0;
+}
+
+// ResidualKernel::boundary_integrals (residual_kernel.hpp:96-169): per patch,
+// in ascending face order, the net mass flux F[0] and the energy flux F[4]
+// (inviscid + viscous) of the boundary faces.  force[3p + k] adds F[1 + k],
+// the momentum flux out of the domain (the force of the fluid on the patch),
+// which the reference does not form.
+int oracle_boundary_integrals(oracle_ctx* c, double* mass, double* energy, double* force, int32_t n_patches) {
+    if (!c) return -1;
+    if (!c->have_residual) {
+        c->err = "boundary_integrals needs a residual";
+        return -3;
+    }
+    if (n_patches != c->npatch) {
+        c->err = "n_patches mismatch";
+        return -1;
+    }
+    for (i32 p = 0; p < n_patches; ++p) {
+        if (mass) mass[p] = 0.0;
+        if (energy) energy[p] = 0.0;
+        if (force) force[3 * p] = force[3 * p + 1] = force[3 * p + 2] = 0.0;
+    }
+    for (i32 f = c->ninner; f < c->nf; ++f) {
+        const size_t b = static_cast<size_t>(f - c->ninner) * 5;
+        const i32 p = c->patch[static_cast<size_t>(f)];
+        if (mass) mass[p] += c->bflux[b];
+        if (energy) energy[p] += c->bflux[b + 4];
+        if (force)
+            for (int k = 0; k < 3; ++k) force[3 * p + k] += c->bflux[b + 1 + static_cast<size_t>(k)];
+    }
     return 0;
 }
 

@@ -108,6 +108,7 @@ int oracle_row_groups(oracle_ctx* ctx, int32_t* colour, int32_t* n_colours, int3
 int oracle_sgs_apply(oracle_ctx* ctx, int32_t sgs_order, const double* r, double* z);
 /* y = A x (AoS block vectors) */
 int oracle_spmv(oracle_ctx* ctx, const double* x, double* y);
+int oracle_boundary_integrals(oracle_ctx* ctx, double* mass, double* energy, double* force, int32_t n_patches);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ def _load():
         "oracle_get_bsr_values": [vp, vp], "oracle_set_bsr_values": [vp, vp],
         "oracle_linear_solve": [vp, vp, vp, vp, vp, vp], "oracle_implicit_step": [vp, vp, vp],
         "oracle_row_groups": [vp, vp, vp, vp, vp, vp, vp], "oracle_sgs_apply": [vp, i32, vp, vp],
-        "oracle_spmv": [vp, vp, vp],
+        "oracle_spmv": [vp, vp, vp], "oracle_boundary_integrals": [vp, vp, vp, vp, i32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -72,6 +72,7 @@ class Oracle:
         nc, nt, nz = C.c_int32(), C.c_int32(), C.c_int64()
         lib.oracle_sizes(h, C.byref(nc), C.byref(nt), C.byref(nz))
         self.n_cells, self.n_total, self.nnzb = nc.value, nt.value, nz.value
+        self.n_patches = len(bcs)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -189,3 +190,10 @@ class Oracle:
         y = np.empty_like(x)
         self._chk(lib.oracle_spmv(self._h, _ptr(x), _ptr(y)))
         return y
+
+    def boundary_integrals(self):
+        """Per patch: net mass flux, energy flux, force (momentum flux, 3)."""
+        n = self.n_patches
+        mass, energy, force = np.empty(n), np.empty(n), np.empty((n, 3))
+        self._chk(lib.oracle_boundary_integrals(self._h, _ptr(mass), _ptr(energy), _ptr(force), n))
+        return mass, energy, force

@@ -274,6 +274,8 @@ void run_components(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryC
     }
     sc::ResidualKernel<EOS, sc::fluxes::HllcFlux, Recon, Phys> rk(mp, aux_conn, aux_geom, eos, phys);
     rk.apply(qc, gxc, gyc, gzc, phc, rp, lam.data());
+    std::vector<double> bnd_mass, bnd_energy;
+    rk.boundary_integrals(qc, gxc, gyc, gzc, phc, bnd_mass, bnd_energy);
 
     std::vector<double> qo, go, po, ro;
     for (int v = 0; v < 5; ++v) qo.insert(qo.end(), q[v].begin(), q[v].end());
@@ -289,6 +291,8 @@ void run_components(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryC
     w.put("phi", po);
     w.put("res", ro);
     w.put("lam", lam);
+    w.put("bnd_mass", bnd_mass);
+    w.put("bnd_energy", bnd_energy);
     w.put("ccn_off", aux_conn.cell_cells_face_offsets);  // face stencil of the limiter
     if (need_grad && scalar<std::int32_t>(r, "gradient") == 2) {
         w.put("lsq_off", aux_conn.cell_cells_node_offsets);

@@ -65,3 +65,22 @@ def test_implicit_steps_bit_exact(name):
     for _ in range(int(fx["steps"][0])):
         o.implicit_step(lp)
     assert np.array_equal(o.get_state().ravel(), fx["ref_steps_U"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_boundary_integrals(name):
+    """Per-patch mass and energy fluxes equal the reference's
+    ResidualKernel::boundary_integrals bit for bit.  The force (momentum flux
+    per patch) has no reference counterpart; it is pinned by telescoping: the
+    interior face fluxes cancel in sum_c R_c, so sum over patches of each
+    boundary integral equals the sum of the cell residuals of that variable."""
+    fx = load(name)
+    mesh, o = _run_oracle(fx)
+    mass, energy, force = o.boundary_integrals()
+    assert np.array_equal(mass, fx["ref_bnd_mass"])
+    assert np.array_equal(energy, fx["ref_bnd_energy"])
+    R, _ = o.residual()
+    S = o.flux_scale()
+    for k, total in ((0, mass.sum()), (1, force[:, 0].sum()), (2, force[:, 1].sum()), (3, force[:, 2].sum()),
+                     (4, energy.sum())):
+        assert abs(total - R[k].sum()) <= 1e-12 * S[k].sum(), (k, total, R[k].sum())

@@ -177,3 +177,34 @@ def test_residual_is_deterministic(name):
     gpu.eval_residual()
     R2, l2 = gpu.residual()
     assert np.array_equal(R1, R2) and np.array_equal(l1, l2)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_boundary_integrals_match_oracle(name):
+    """Per-patch mass, energy and force: |gpu - oracle| <= 1e-12 x (sum over
+    all boundary faces of |F|) -- the device sums each patch in a fixed tree
+    order instead of the reference's face order."""
+    case = CASES[name]
+    mesh, gpu, ora = _pair(case)
+    gpu.eval_residual()
+    ora.eval_residual()
+    mg, eg, fg = gpu.boundary_integrals()
+    mo, eo, fo = ora.boundary_integrals()
+    S = ora.flux_scale().sum(axis=1)
+    assert np.all(np.abs(mg - mo) <= 1e-12 * S[0] + 1e-300)
+    assert np.all(np.abs(eg - eo) <= 1e-12 * S[4] + 1e-300)
+    for k in range(3):
+        assert np.all(np.abs(fg[:, k] - fo[:, k]) <= 1e-12 * S[1 + k] + 1e-300)
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_boundary_integrals_match_reference_golden(name):
+    fx = load(name)
+    mesh = mesh_of(fx)
+    gpu = cfdg.Solver(mesh, numerics_of(fx), bcs_of(fx))
+    gpu.set_state(fx["U"])
+    gpu.eval_residual()
+    mg, eg, _ = gpu.boundary_integrals()
+    scale = np.abs(fx["ref_res"]).reshape(5, -1).sum(axis=1)
+    assert np.all(np.abs(mg - fx["ref_bnd_mass"]) <= 1e-12 * scale[0] + 1e-300)
+    assert np.all(np.abs(eg - fx["ref_bnd_energy"]) <= 1e-12 * scale[4] + 1e-300)
