@@ -745,6 +745,7 @@ void implicit_step(cfdg_ctx& c, const cfdg_linparams& lp, cfdg_step_result* out)
     c.sync();
     c.have_residual = false;
     c.have_dt = false;
+    c.have_prev_du = true;
     for (int v = 0; v < 5; ++v) out->l2[v] = l2[v];
     out->linear = lr;
 }
@@ -873,6 +874,7 @@ int cfdg_set_state(cfdg_ctx* c, const double* U, int64_t ld) {
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
+        c->have_prev_du = false;
     });
 }
 
@@ -902,6 +904,7 @@ int cfdg_init_uniform(cfdg_ctx* c, double rho, const double vel[3], double p) {
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
+        c->have_prev_du = false;
         // the uniform state is the freestream of the learned warm start's normalisation
         c->ref_rho = rho;
         c->ref_p = p;
@@ -1030,6 +1033,7 @@ int cfdg_linear_solve(cfdg_ctx* c, const cfdg_linparams* lp, const double* b, co
     return guarded(c, [&] {
         if (!lp) throw CfdgError(CFDG_ERR_ARG, "null linear parameters");
         linear_solve(*c, *lp, b, x0, x, res);
+        c->have_prev_du = false;
     });
 }
 

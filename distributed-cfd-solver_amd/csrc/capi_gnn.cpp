@@ -262,7 +262,8 @@ void ensure_ctx_graph(cfdg_ctx& c) {
     for (int i = 0; i < c.nc; ++i) s += std::cbrt(vol[static_cast<std::size_t>(i)]);
     c.h_mean = c.nc > 0 ? s / c.nc : 1.0;
     c.gx.alloc(static_cast<std::size_t>(c.nc) * gn::kF);
-    c.gnorm.alloc(1 + 512);
+    c.gnorm.alloc(2 + 512);
+    c.gnorm.zero(c.stream);
     c.sync();
     c.have_graph = true;
 }
@@ -280,9 +281,11 @@ void gnn_step_features(cfdg_ctx& c) {
     if (!c.gnn_record && !c.warm) return;
     ensure_ctx_graph(c);
     const gn::Ref r = make_ref(c);
-    gn::launch_explicit_norm(c.nc, c.res.get(), c.lam.get(), r, c.gnorm.get() + 1, c.gnorm.get(), c.stream);
-    gn::launch_features(c.nc, c.cs.get(), c.ld, c.res.get(), c.lam.get(), c.dt.get(), r, c.gnorm.get(), c.gx.get(),
-                        c.stream);
+    // keep the previous step's norm in gnorm[1], then form this step's in gnorm[0]
+    hip_check(hipMemcpyAsync(c.gnorm.get() + 1, c.gnorm.get(), sizeof(double), hipMemcpyDeviceToDevice, c.stream), "copy");
+    gn::launch_explicit_norm(c.nc, c.res.get(), c.lam.get(), r, c.gnorm.get() + 2, c.gnorm.get(), c.stream);
+    gn::launch_features(c.nc, c.cs.get(), c.ld, c.res.get(), c.lam.get(), c.dt.get(), r, c.gnorm.get(), c.x.get(),
+                        c.have_prev_du ? 1 : 0, c.gx.get(), c.stream);
     c.check_launch();
     c.have_features = true;
 }

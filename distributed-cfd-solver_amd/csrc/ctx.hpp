@@ -129,7 +129,8 @@ struct cfdg_ctx {
     DBuf<float> geg;
     double h_mean = 0.0;
     DBuf<cfdg::bf16> gx;                // features of the current step
-    DBuf<double> gnorm;                 // [0] squared norm of the explicit update, then partials
+    DBuf<double> gnorm;                 // [0] squared norm of -R / lambda, [1] the previous step's, then partials
+    bool have_prev_du = false;          // x holds the previous implicit step's update
     bool have_features = false;
     long long warm_used = 0, warm_rejected = 0;
     // timing

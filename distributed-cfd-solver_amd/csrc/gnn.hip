@@ -328,7 +328,7 @@ __device__ __forceinline__ double rms_of(const double* rnorm2, int nc) {
 
 __global__ void __launch_bounds__(kBlock) k_features(int nc, const double* cs, long long ld, const double* res,
                                                      const double* lam, const double* dt, Ref ref, const double* rnorm2,
-                                                     bf16* x) {
+                                                     const double* du_prev, int have_prev, bf16* x) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const double rn = rms_of(rnorm2, nc);
@@ -349,6 +349,13 @@ __global__ void __launch_bounds__(kBlock) k_features(int nc, const double* cs, l
     f[12] = static_cast<float>(log(h / ref.h_mean));
     f[13] = static_cast<float>(log10(dt[c] * (ref.speed + ref.a) / h) / 3.0);
     f[14] = 1.f;
+    if (have_prev) {
+        for (int v = 0; v < 5; ++v)
+            f[15 + v] = static_cast<float>(du_prev[static_cast<long long>(c) * 5 + v] / (ref.s[v] * rn));
+        const double rp = sqrt(rnorm2[1] / (5.0 * static_cast<double>(nc)));
+        f[20] = static_cast<float>(rp > 1.0e-300 ? log(rn / rp) : 0.0);
+        f[21] = 1.f;
+    }
     bf16* xo = x + static_cast<long long>(c) * kF;
     for (int k = 0; k < kF; ++k) xo[k] = bf16_from_float(f[k]);
 }
@@ -389,9 +396,10 @@ void launch_explicit_norm(int nc, const double* res, const double* lam, Ref ref,
 }
 
 void launch_features(int nc, const double* cs, long long ld, const double* res, const double* lam, const double* dt,
-                     Ref ref, const double* rnorm2, bf16* x, hipStream_t s) {
+                     Ref ref, const double* rnorm2, const double* du_prev, int have_prev, bf16* x, hipStream_t s) {
     if (nc > 0)
-        hipLaunchKernelGGL(k_features, grid_for(nc), dim3(kBlock), 0, s, nc, cs, ld, res, lam, dt, ref, rnorm2, x);
+        hipLaunchKernelGGL(k_features, grid_for(nc), dim3(kBlock), 0, s, nc, cs, ld, res, lam, dt, ref, rnorm2,
+                           du_prev, have_prev, x);
 }
 
 void launch_targets(int nc, const double* du, Ref ref, const double* rnorm2, float* t, hipStream_t s) {

@@ -8,7 +8,7 @@
 // neighbour, a boundary face is a self loop.  Edge geometry g_e = (n_x, n_y,
 // n_z, A / V_i^(2/3)) with the normal pointing away from node i.
 //
-// Network (F = 32 input features, 16 used; H = 64; G = 4; O = 5):
+// Network (F = 32 input features, 24 used; H = 64; G = 4; O = 5):
 //   z1_e   = WP1 x_i + WQ1 x_j + G1 g_e + b1          e = (i -> j)
 //   h1_i   = mean_{e in adj(i)} relu(z1_e)
 //   z2_e   = WP2 h1_i + WQ2 h1_j + G2 g_e + b2
@@ -30,7 +30,7 @@ namespace cfdg {
 namespace gnn {
 
 constexpr int kF = 32;   // stored input features (bf16); kFUsed carry data, the rest are zero
-constexpr int kFUsed = 16;
+constexpr int kFUsed = 24;
 constexpr int kH = 64;
 constexpr int kG = 4;
 constexpr int kO = 5;
@@ -83,9 +83,17 @@ void launch_build_graph(int nc, int ninner, const int* cf_off, const int* cf, co
 // sum over cells and variables of (R_v / lambda / s_v)^2 into out[0] (double, deterministic)
 void launch_explicit_norm(int nc, const double* res, const double* lam, Ref ref, double* partial, double* out,
                           hipStream_t s);
-// features (bf16, n x kF); rnorm2 = device pointer to the sum written by launch_explicit_norm
+// features (bf16, n x kF); rnorm2 = device pointer to the sum written by
+// launch_explicit_norm, rnorm2[1] = the previous step's sum; du_prev = the
+// previous step's update (AoS, device order) when have_prev, else ignored.
+//   0-4   p / p_inf - 1, u / U, v / U, w / U, T / T_inf - 1
+//   5-9   -R / lambda / s, divided by its RMS over the mesh (rnorm)
+//   10    log10(rnorm) / 8         11  freestream Mach
+//   12    ln(V^(1/3) / mean)       13  log10(dt (U + a) / V^(1/3)) / 3
+//   14    1                        15-19 previous update / (s rnorm)
+//   20    ln(rnorm / previous rnorm)  21  1 when a previous update exists
 void launch_features(int nc, const double* cs, long long ld, const double* res, const double* lam, const double* dt,
-                     Ref ref, const double* rnorm2, bf16* x, hipStream_t s);
+                     Ref ref, const double* rnorm2, const double* du_prev, int have_prev, bf16* x, hipStream_t s);
 // targets t = du / (s rnorm) (du AoS, 5 per cell) and the inverse map x0 = y s rnorm
 void launch_targets(int nc, const double* du, Ref ref, const double* rnorm2, float* t, hipStream_t s);
 void launch_prediction(int nc, const float* y, Ref ref, const double* rnorm2, double* x0, hipStream_t s);

@@ -328,7 +328,7 @@ int cfdg_gnn_create(uint64_t seed, cfdg_gnn** out);
 int cfdg_gnn_destroy(cfdg_gnn* gnn);
 const char* cfdg_gnn_last_error(const cfdg_gnn* gnn);
 int cfdg_gnn_num_params(int32_t* n);
-/* Flat FP32 parameters in the order [WP1; WQ1] (128 x 32, columns >= 16
+/* Flat FP32 parameters in the order [WP1; WQ1] (128 x 32, columns >= 24
    zero), G1 (64 x 4), b1 (64), [WP2; WQ2] (128 x 64), G2 (64 x 4), b2 (64),
    Wo (5 x 64), bo (5); all row-major.  set_params resets the Adam state. */
 int cfdg_gnn_get_params(cfdg_gnn* gnn, float* params, int32_t n);

@@ -19,7 +19,7 @@ tests/test_gnn_oracle.py.
 """
 import numpy as np
 
-F, FUSED, H, G, O = 32, 16, 64, 4, 5
+F, FUSED, H, G, O = 32, 24, 64, 4, 5
 PQ = 2 * H
 SIZES = [("W1", (PQ, F)), ("G1", (H, G)), ("b1", (H,)), ("W2", (PQ, H)), ("G2", (H, G)), ("b2", (H,)),
          ("Wo", (O, H)), ("bo", (O,))]

@@ -393,7 +393,8 @@ struct BcParams {  // resolved per-patch parameters (bc_manager.hpp:235-352)
 struct oracle_ctx {
     std::string err;
     // mesh
-    i32 nc = 0, nf = 0, ninner = 0, nbf = 0, ntot = 0, npatch = 0;
+    i32 nc = 0, nf = 0, ninner = 0, nbf = 0, ntot = 0, npatch = 0, n_nodes = 0;
+    std::vector<int> patch_type;  // boundary condition type of each patch
     std::vector<i32> owner, neigh, patch, cell_nodes_off, cell_nodes;
     std::vector<double> cx, cy, cz, vol, fx, fy, fz, nx, ny, nz, area;
     // numerics
@@ -1444,6 +1445,9 @@ int oracle_create(const oracle_mesh_view* m, const oracle_numerics* num, const o
             bp.fend = e;
             c->bcs.push_back(bp);
         }
+        c->n_nodes = m->n_nodes;
+        c->patch_type.assign(static_cast<size_t>(c->npatch), -1);
+        for (i32 p = 0; p < n_bcs && p < c->npatch; ++p) c->patch_type[static_cast<size_t>(p)] = bcs[p].type;
         build_cell_faces(*c);
         build_cell_cells_face(*c);
         if (num->gradient == 2) {
@@ -1728,6 +1732,59 @@ int oracle_boundary_integrals(oracle_ctx* c, double* mass, double* energy, doubl
         if (force)
             for (int k = 0; k < 3; ++k) force[3 * p + k] += c->bflux[b + 1 + static_cast<size_t>(k)];
     }
+    return 0;
+}
+
+// MeshAuxGeometry::add_wall_distance + relax_distance_graph
+// (aux_geometry.cpp:545-697): cells owning a no-slip wall face start at the
+// distance to its centroid, every other cell at 1e30; then forward and backward
+// Gauss-Seidel sweeps d_c = min(d_c, d_n + |x_n - x_c|) over the face (stencil
+// 0) or vertex (stencil 1) neighbours until the largest relative decrease is
+// <= rel_tol or max_sweeps sweeps have run.
+int oracle_wall_distance(oracle_ctx* c, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist) {
+    if (!c || !dist) return -1;
+    const double kInf = 1.0e30, kMinDist = 1.0e-12;
+    std::vector<double> d(static_cast<size_t>(c->nc), kInf);
+    for (i32 f = c->ninner; f < c->nf; ++f) {
+        const size_t fs = static_cast<size_t>(f);
+        const int t = c->patch_type[static_cast<size_t>(c->patch[fs])];
+        if (t != 5 && t != 6) continue;  // NO_SLIP_WALL, NO_SLIP_WALL_HEAT_FLUX
+        const size_t c0 = static_cast<size_t>(c->owner[fs]);
+        const double ex = c->fx[fs] - c->cx[c0], ey = c->fy[fs] - c->cy[c0], ez = c->fz[fs] - c->cz[c0];
+        d[c0] = std::min(d[c0], std::sqrt(ex * ex + ey * ey + ez * ez));
+    }
+    if (stencil == 1 && c->ccn_off.empty()) build_cell_cells_node(*c, c->n_nodes);
+    const std::vector<i32>& off = (stencil == 1) ? c->ccn_off : c->ccf_off;
+    const std::vector<i32>& nb = (stencil == 1) ? c->ccn : c->ccf;
+    std::vector<double> ed(nb.size());
+    for (i32 i = 0; i < c->nc; ++i) {
+        const size_t is = static_cast<size_t>(i);
+        for (i32 j = off[is]; j < off[is + 1]; ++j) {
+            const size_t n = static_cast<size_t>(nb[static_cast<size_t>(j)]);
+            const double dx = c->cx[n] - c->cx[is], dy = c->cy[n] - c->cy[is], dz = c->cz[n] - c->cz[is];
+            ed[static_cast<size_t>(j)] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        double max_rel = 0.0;
+        const auto relax = [&](size_t cell) {
+            const double d0 = d[cell];
+            double dn = d0;
+            for (i32 j = off[cell]; j < off[cell + 1]; ++j) {
+                const double cand = d[static_cast<size_t>(nb[static_cast<size_t>(j)])] + ed[static_cast<size_t>(j)];
+                dn = (cand < dn) ? cand : dn;
+            }
+            if (dn < d0) {
+                const double rel = (d0 - dn) / std::max(d0, kMinDist);
+                max_rel = (rel > max_rel) ? rel : max_rel;
+                d[cell] = dn;
+            }
+        };
+        for (size_t cell = 0; cell < static_cast<size_t>(c->nc); ++cell) relax(cell);
+        for (size_t cell = static_cast<size_t>(c->nc); cell-- > 0;) relax(cell);
+        if (max_rel <= rel_tol) break;
+    }
+    std::copy(d.begin(), d.end(), dist);
     return 0;
 }
 

@@ -109,6 +109,7 @@ int oracle_sgs_apply(oracle_ctx* ctx, int32_t sgs_order, const double* r, double
 /* y = A x (AoS block vectors) */
 int oracle_spmv(oracle_ctx* ctx, const double* x, double* y);
 int oracle_boundary_integrals(oracle_ctx* ctx, double* mass, double* energy, double* force, int32_t n_patches);
+int oracle_wall_distance(oracle_ctx* ctx, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,7 @@ def _load():
         "oracle_linear_solve": [vp, vp, vp, vp, vp, vp], "oracle_implicit_step": [vp, vp, vp],
         "oracle_row_groups": [vp, vp, vp, vp, vp, vp, vp], "oracle_sgs_apply": [vp, i32, vp, vp],
         "oracle_spmv": [vp, vp, vp], "oracle_boundary_integrals": [vp, vp, vp, vp, i32],
+        "oracle_wall_distance": [vp, i32, i32, d, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -190,6 +191,11 @@ class Oracle:
         y = np.empty_like(x)
         self._chk(lib.oracle_spmv(self._h, _ptr(x), _ptr(y)))
         return y
+
+    def wall_distance(self, stencil=1, max_sweeps=500, rel_tol=1e-8):
+        d = np.empty(self.n_cells)
+        self._chk(lib.oracle_wall_distance(self._h, stencil, max_sweeps, rel_tol, _ptr(d)))
+        return d
 
     def boundary_integrals(self):
         """Per patch: net mass flux, energy flux, force (momentum flux, 3)."""

@@ -293,6 +293,19 @@ void run_components(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryC
     w.put("lam", lam);
     w.put("bnd_mass", bnd_mass);
     w.put("bnd_energy", bnd_energy);
+    // wall distance from the no-slip patches (MeshAuxGeometry::add_wall_distance), both stencils
+    {
+        const auto types = get<std::int32_t>(r, "bc_type");
+        std::vector<bool> wall(mp.patches.size(), false);
+        for (std::size_t p = 0; p < wall.size() && p < types.size(); ++p)
+            wall[p] = types[p] == 5 || types[p] == 6;  // NO_SLIP_WALL, NO_SLIP_WALL_HEAT_FLUX
+        mesh::MeshAuxConnectivity wc;
+        mesh::MeshAuxGeometry wf, wn;
+        wf.add_wall_distance(mp, wc, MPI_COMM_WORLD, wall, mesh::WallDistStencil::FaceNeighbors, 100, 1.0e-4);
+        wn.add_wall_distance(mp, wc, MPI_COMM_WORLD, wall, mesh::WallDistStencil::NodeNeighbors, 500, 1.0e-8);
+        w.put("wall_dist_face", wf.cell_wall_dist);
+        w.put("wall_dist_node", wn.cell_wall_dist);
+    }
     w.put("ccn_off", aux_conn.cell_cells_face_offsets);  // face stencil of the limiter
     if (need_grad && scalar<std::int32_t>(r, "gradient") == 2) {
         w.put("lsq_off", aux_conn.cell_cells_node_offsets);

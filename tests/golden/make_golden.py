@@ -98,8 +98,8 @@ def make(name, case, steps, seed=11):
     fx["init"] = np.array([rho, u, v, w, p])
     fx["lin"] = np.array([lp.preconditioner, lp.sgs_order, lp.rel_tol, lp.abs_tol, lp.max_iter], dtype=np.float64)
     fx["steps"] = np.array([steps])
-    for k in ("q", "grad", "phi", "res", "lam", "bnd_mass", "bnd_energy", "dt", "alpha", "row_ptr", "cols", "diag",
-              "values", "du", "lin_status", "lin_norms", "steps_U"):
+    for k in ("q", "grad", "phi", "res", "lam", "bnd_mass", "bnd_energy", "wall_dist_face", "wall_dist_node", "dt",
+              "alpha", "row_ptr", "cols", "diag", "values", "du", "lin_status", "lin_norms", "steps_U"):
         fx[f"ref_{k}"] = out[k]
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **fx)

@@ -84,3 +84,33 @@ def test_boundary_integrals(name):
     for k, total in ((0, mass.sum()), (1, force[:, 0].sum()), (2, force[:, 1].sum()), (3, force[:, 2].sum()),
                      (4, energy.sum())):
         assert abs(total - R[k].sum()) <= 1e-12 * S[k].sum(), (k, total, R[k].sum())
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_wall_distance_bit_exact(name):
+    """Wall distance from the no-slip patches equals the reference's
+    MeshAuxGeometry::add_wall_distance bit for bit, with the face-neighbour
+    stencil (100 sweeps, 1e-4) and the vertex-neighbour one (500, 1e-8)."""
+    fx = load(name)
+    mesh, o = _run_oracle(fx)
+    assert np.array_equal(o.wall_distance(0, 100, 1e-4), fx["ref_wall_dist_face"])
+    assert np.array_equal(o.wall_distance(1, 500, 1e-8), fx["ref_wall_dist_node"])
+
+
+def test_wall_distance_closed_form():
+    """On an undistorted hex box with a no-slip wall at y = 0 the face-stencil
+    graph distance of a cell is exactly its centroid height (straight column
+    of cells), a closed form independent of the reference."""
+    import cfdg
+    import cfdg.cases as cases
+    case = cases.plate(ni=6, nj=5, nk=2, jitter=0.0, reorder=0)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    bcs = list(case.bcs)
+    for b in bcs:
+        b.type = 7  # symmetry everywhere ...
+    bcs[2].type = 5  # ... but a no-slip wall on ymin (x >= 0)
+    bcs[6].type = 5  # and on ymin (x < 0)
+    o = Oracle(mesh, case.numerics, bcs)
+    d = o.wall_distance(0, 100, 1e-12)
+    cy = mesh.array("cell_cy")
+    assert np.allclose(d, cy, rtol=0, atol=1e-14)

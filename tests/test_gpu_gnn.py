@@ -49,11 +49,11 @@ def test_loss_and_gradient_match_cpu_oracle():
     case, mesh, s, g = _recorded(steps=2)
     assert g.num_samples() == 2
     p = g.params().astype(np.float64)
-    assert np.all(p.reshape(-1)[: 128 * 32].reshape(128, 32)[:, 16:] == 0.0)
+    assert np.all(p.reshape(-1)[: 128 * 32].reshape(128, 32)[:, gnn_ref.FUSED:] == 0.0)
     for k in range(2):
         loss, grad = g.loss_grad(k)
         smp = g.sample(k)
-        assert np.all(smp["x"][:, 16:] == 0.0)
+        assert np.all(smp["x"][:, gnn_ref.FUSED:] == 0.0)
         ref_loss, ref_grad = gnn_ref.loss_grad(p, smp["x"], smp["t"], smp["adj_off"], smp["adj"], smp["rev"],
                                                smp["eg"].astype(np.float64), quant=True)
         assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss), (loss, ref_loss)
