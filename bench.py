@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--implicit-steps", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "off"], default="auto")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--drop-size", default="64x32x16",
+                    help="plate mesh of the time-to-1e-8-drop runs ('off' skips them)")
+    ap.add_argument("--drop-max-steps", type=int, default=12000)
     ap.add_argument("--pmc-run", action="store_true",
                     help="profiling mode: one 1 GiB device copy (counter calibration) and flux-kernel launches only")
     return ap.parse_args()
@@ -177,7 +180,81 @@ def main():
         "implicit_step": imp,
         "setup_s": {"mesh": round(t_mesh, 2), "context": round(t_setup, 2)},
     }
+    if a.drop_size != "off":
+        solver.close()
+        out["time_to_drop"] = time_to_drop(a)
     print(json.dumps(out))
+
+
+def time_to_drop(a):
+    """Wall time to a 1e-8 drop of the reference's relative residual
+    (max_v L2_v / L2_v of the first step, solver.hpp:740-765) on a steady
+    laminar plate, for the plain FP64 path, the FP32-Krylov form and the FP32
+    form with the learned warm start.  Second-order unlimited MUSCL with the
+    node-LSQ gradient (the Venkatakrishnan limiter stalls this case near 1e-6),
+    CFL 1000, from the freestream with seeded 2 % noise.  The network is
+    trained first on 8 seeded 32x16x8 plates with Mach drawn from U(0.2, 0.5)
+    (20 recorded steps each); its training time is reported separately."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    ni, nj, nk = (int(x) for x in a.drop_size.split("x"))
+    kw = dict(limiter="UNLIMITED", cfl=1000.0)
+    lp = cases.default_linparams()
+    fp32 = cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1)
+
+    def start(case, seed):
+        mesh = cfdg.Mesh.generate(**case.mesh)
+        s = cfdg.Solver(mesh, case.numerics, case.bcs)
+        s.init_uniform(case.rho, case.velocity, case.p)
+        U = s.get_state()
+        s.set_state(U * (1.0 + 0.02 * np.random.default_rng(seed).standard_normal(U.shape)))
+        return s
+
+    def run(form, gnn=None):
+        case = cases.plate(ni=ni, nj=nj, nk=nk, **kw)
+        s = start(case, 1)
+        if form != "plain":
+            s.set_precision(fp32)
+        if gnn is not None:
+            s.set_warm_start(gnn)
+        r = s.run_steady(lp, a.drop_max_steps, rel_tol=1e-8)
+        res = {"seconds": round(r.seconds, 3), "steps": int(r.steps), "linear_iterations": int(r.linear_iterations),
+               "converged": r.status == cfdg.CONVERGED, "rel": r.rel}
+        if gnn is not None:
+            res["warm_start_kept_rejected"] = list(s.warm_start_stats())
+        U = s.get_state()
+        s.close()
+        return res, U
+
+    out = {"workload": f"laminar plate {ni}x{nj}x{nk} hexahedra, Mach 0.3, Re_L 1e5, unlimited MUSCL + node LSQ, "
+                       "CFL 1000, BiCGSTAB + multicolour SGS (rel 0.1), 2% seeded noise on the freestream"}
+    out["plain_fp64"], U0 = run("plain")
+    out["fp32_krylov"], U1 = run("fp32")
+    t0 = time.perf_counter()
+    gnn = cfdg.Gnn(seed=11)
+    rng = np.random.default_rng(2024)
+    for m in range(8):
+        case = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m, **kw)
+        s = start(case, m)
+        s.set_recording(True)
+        for it in range(40):
+            s.implicit_step(lp)
+            if it % 2 == 1:
+                gnn.record(s)
+        s.close()
+    loss = gnn.train(epochs=30, lr=1e-3, seed=5)
+    out["training"] = {"seconds": round(time.perf_counter() - t0, 2), "samples": gnn.num_samples(),
+                       "final_loss": loss}
+    out["fp32_krylov_learned"], U2 = run("learned", gnn)
+    scale = np.abs(U0).max(axis=1, keepdims=True)
+    out["state_error_vs_fp64"] = {"fp32_krylov": float(np.max(np.abs(U1 - U0) / scale)),
+                                  "fp32_krylov_learned": float(np.max(np.abs(U2 - U0) / scale))}
+    out["speedup_mixed_learned"] = round(out["plain_fp64"]["seconds"] / out["fp32_krylov_learned"]["seconds"], 3)
+    out["speedup_mixed"] = round(out["plain_fp64"]["seconds"] / out["fp32_krylov"]["seconds"], 3)
+    return out
 
 
 def cpu_baseline(a, mesh, case, U, nc):

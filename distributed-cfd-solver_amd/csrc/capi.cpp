@@ -588,7 +588,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         std::vector<double> dx, dy, dz, di;
         cfdg::face_distances(m, dx, dy, dz, di);
         fph.assign(z(c.fld) * cfdg::kFacePlanes, 0.0);
-        const auto put = [&](int k, int f, double v) { fph[z(k) * z(c.fld) + z(f)] = v; };
+        const auto put = [&](int k, int f, double v) { fph[z(cfdg::plane_at(k, c.fld, f))] = v; };
         for (int f = 0; f < c.nf; ++f) {
             put(cfdg::kFN, f, m.face_nx[z(f)]);
             put(cfdg::kFN + 1, f, m.face_ny[z(f)]);
@@ -630,7 +630,8 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         std::vector<int2> tcells(z(nte));
         for (long long e = 0; e < nte; ++e) {
             const i32 f = tf.faces[z(e)];
-            for (int k = 0; k < cfdg::kFacePlanes; ++k) tph[z(k) * z(c.tld) + z(e)] = fph[z(k) * z(c.fld) + z(f)];
+            for (int k = 0; k < cfdg::kFacePlanes; ++k)
+                tph[z(cfdg::plane_at(k, c.tld, e))] = fph[z(cfdg::plane_at(k, c.fld, f))];
             tcells[z(e)] = fcells[z(f)];
         }
         c.tf.upload(tph, s);
@@ -662,7 +663,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     {
         c.ld = (static_cast<long long>(c.ntot) + 63) / 64 * 64;
         std::vector<double> cp(z(c.ld) * cfdg::kCellPlanes, 0.0);
-        const auto put = [&](int k, int r, double v) { cp[z(k) * z(c.ld) + z(r)] = v; };
+        const auto put = [&](int k, int r, double v) { cp[z(cfdg::plane_at(k, c.ld, r))] = v; };
         for (int r = 0; r < c.nc; ++r) {
             const std::size_t o = z(c.new2old[z(r)]);
             put(cfdg::kRC, r, m.cell_cx[o]);
@@ -926,7 +927,7 @@ int cfdg_get_primitives(cfdg_ctx* c, double* q, int64_t ld) {
         const auto h = c->download_planes();
         for (int r = 0; r < c->ntot; ++r) {
             const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
-            for (int v = 0; v < 5; ++v) q[v * ld + o] = h[z(cfdg::kRQ + v) * z(c->ld) + z(r)];
+            for (int v = 0; v < 5; ++v) q[v * ld + o] = h[z(cfdg::plane_at(cfdg::kRQ + v, c->ld, r))];
         }
     });
 }
@@ -936,7 +937,7 @@ int cfdg_get_gradients(cfdg_ctx* c, double* grad, int64_t ld, double* phi, int64
         if (grad && ld < c->ntot) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld < n_total");
         if (phi && ld_phi < c->nc) throw CfdgError(CFDG_ERR_ARG, "get_gradients: ld_phi < n_cells");
         const auto h = c->download_planes();
-        const auto at = [&](int k, int r) { return h[z(k) * z(c->ld) + z(r)]; };
+        const auto at = [&](int k, int r) { return h[z(cfdg::plane_at(k, c->ld, r))]; };
         for (int r = 0; r < c->ntot; ++r) {
             const long long o = (r < c->nc) ? c->new2old[z(r)] : r;
             if (grad)

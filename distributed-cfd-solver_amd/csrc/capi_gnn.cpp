@@ -255,15 +255,14 @@ void ensure_ctx_graph(cfdg_ctx& c) {
                            c.ld, c.gadj.get(), c.grev.get(), c.geg.get(), c.stream);
     c.check_launch();
     // mean V^(1/3) (host, from the static volumes)
-    std::vector<double> vol(static_cast<std::size_t>(c.nc));
-    c.d2h(vol.data(), c.cs.get() + cfdg::kRVol * c.ld, vol.size());
-    c.sync();
+    const std::vector<double> planes = c.download_planes();
     double s = 0.0;
-    for (int i = 0; i < c.nc; ++i) s += std::cbrt(vol[static_cast<std::size_t>(i)]);
+    for (int i = 0; i < c.nc; ++i) s += std::cbrt(planes[static_cast<std::size_t>(cfdg::plane_at(cfdg::kRVol, c.ld, i))]);
     c.h_mean = c.nc > 0 ? s / c.nc : 1.0;
     c.gx.alloc(static_cast<std::size_t>(c.nc) * gn::kF);
     c.gnorm.alloc(2 + 512);
     c.gnorm.zero(c.stream);
+    c.du_prev.alloc(static_cast<std::size_t>(c.nc) * 5);
     c.sync();
     c.have_graph = true;
 }
@@ -281,11 +280,17 @@ void gnn_step_features(cfdg_ctx& c) {
     if (!c.gnn_record && !c.warm) return;
     ensure_ctx_graph(c);
     const gn::Ref r = make_ref(c);
-    // keep the previous step's norm in gnorm[1], then form this step's in gnorm[0]
+    // keep the previous step's norm in gnorm[1] and its update in du_prev, then
+    // form this step's norm in gnorm[0]
     hip_check(hipMemcpyAsync(c.gnorm.get() + 1, c.gnorm.get(), sizeof(double), hipMemcpyDeviceToDevice, c.stream), "copy");
+    c.features_had_prev = c.have_prev_du;
+    if (c.have_prev_du)
+        hip_check(hipMemcpyAsync(c.du_prev.get(), c.x.get(), sizeof(double) * 5 * static_cast<std::size_t>(c.nc),
+                                 hipMemcpyDeviceToDevice, c.stream),
+                  "copy");
     gn::launch_explicit_norm(c.nc, c.res.get(), c.lam.get(), r, c.gnorm.get() + 2, c.gnorm.get(), c.stream);
-    gn::launch_features(c.nc, c.cs.get(), c.ld, c.res.get(), c.lam.get(), c.dt.get(), r, c.gnorm.get(), c.x.get(),
-                        c.have_prev_du ? 1 : 0, c.gx.get(), c.stream);
+    gn::launch_features(c.nc, c.cs.get(), c.ld, c.res.get(), c.lam.get(), c.dt.get(), r, c.gnorm.get(),
+                        c.du_prev.get(), c.features_had_prev ? 1 : 0, c.gx.get(), c.stream);
     c.check_launch();
     c.have_features = true;
 }
@@ -297,7 +302,8 @@ bool gnn_warm_guess(cfdg_ctx& c, double* x0) {
     c.sync();
     forward(g, ctx_graph(c), c.gx.get());
     g.sync();
-    gn::launch_prediction(c.nc, g.y.get(), make_ref(c), c.gnorm.get(), x0, c.stream);
+    gn::launch_prediction(c.nc, g.y.get(), c.du_prev.get(), c.features_had_prev ? 1 : 0, make_ref(c), c.gnorm.get(), x0,
+                          c.stream);
     c.check_launch();
     return true;
 }
@@ -418,7 +424,8 @@ int cfdg_gnn_record(cfdg_gnn* g, cfdg_ctx* c) {
         smp->x.alloc(n * gn::kF);
         smp->t.alloc(n * gn::kO);
         hip_check(hipMemcpy(smp->x.get(), c->gx.get(), n * gn::kF * sizeof(bf16), hipMemcpyDeviceToDevice), "copy");
-        gn::launch_targets(c->nc, c->x.get(), make_ref(*c), c->gnorm.get(), smp->t.get(), c->stream);
+        gn::launch_targets(c->nc, c->x.get(), c->du_prev.get(), c->features_had_prev ? 1 : 0, make_ref(*c),
+                           c->gnorm.get(), smp->t.get(), c->stream);
         c->check_launch();
         c->sync();
         g->samples.push_back(std::move(smp));

@@ -131,6 +131,8 @@ struct cfdg_ctx {
     DBuf<cfdg::bf16> gx;                // features of the current step
     DBuf<double> gnorm;                 // [0] squared norm of -R / lambda, [1] the previous step's, then partials
     bool have_prev_du = false;          // x holds the previous implicit step's update
+    DBuf<double> du_prev;               // the previous step's update, copied when the features are formed
+    bool features_had_prev = false;     // du_prev was valid for the current features
     bool have_features = false;
     long long warm_used = 0, warm_rejected = 0;
     // timing

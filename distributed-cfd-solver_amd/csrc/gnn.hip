@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_build_graph(int nc, int ninner, cons
                                                         const double* cs, long long ld, int* adj, int* rev, float* eg) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const double vol = cs[kRVol * ld + c];
+    const double vol = cs[plane_at(kRVol, ld, c)];
     const double v23 = cbrt(vol) * cbrt(vol);
     for (int e = cf_off[c]; e < cf_off[c + 1]; ++e) {
         const int f = cf[e];
@@ -360,21 +360,30 @@ __global__ void __launch_bounds__(kBlock) k_features(int nc, const double* cs, l
     for (int k = 0; k < kF; ++k) xo[k] = bf16_from_float(f[k]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_targets(int nc, const double* du, Ref ref, const double* rnorm2, float* t) {
+// The network predicts the change of the update: du_n = du_{n-1} + y s rnorm
+// (du_{n-1} = 0 on the first step), a skip connection carried in FP64.
+__global__ void __launch_bounds__(kBlock) k_targets(int nc, const double* du, const double* du_prev, int have_prev,
+                                                    Ref ref, const double* rnorm2, float* t) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const double rn = rms_of(rnorm2, nc);
-    for (int v = 0; v < 5; ++v)
-        t[static_cast<long long>(c) * kO + v] = static_cast<float>(du[static_cast<long long>(c) * 5 + v] / (ref.s[v] * rn));
+    for (int v = 0; v < 5; ++v) {
+        const long long i = static_cast<long long>(c) * 5 + v;
+        const double d = have_prev ? du[i] - du_prev[i] : du[i];
+        t[static_cast<long long>(c) * kO + v] = static_cast<float>(d / (ref.s[v] * rn));
+    }
 }
 
-__global__ void __launch_bounds__(kBlock) k_prediction(int nc, const float* y, Ref ref, const double* rnorm2,
-                                                       double* x0) {
+__global__ void __launch_bounds__(kBlock) k_prediction(int nc, const float* y, const double* du_prev, int have_prev,
+                                                       Ref ref, const double* rnorm2, double* x0) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const double rn = rms_of(rnorm2, nc);
-    for (int v = 0; v < 5; ++v)
-        x0[static_cast<long long>(c) * 5 + v] = static_cast<double>(y[static_cast<long long>(c) * kO + v]) * ref.s[v] * rn;
+    for (int v = 0; v < 5; ++v) {
+        const long long i = static_cast<long long>(c) * 5 + v;
+        const double corr = static_cast<double>(y[static_cast<long long>(c) * kO + v]) * ref.s[v] * rn;
+        x0[i] = have_prev ? du_prev[i] + corr : corr;
+    }
 }
 
 inline dim3 grid_for(long long n) { return dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)); }
@@ -402,12 +411,16 @@ void launch_features(int nc, const double* cs, long long ld, const double* res, 
                            du_prev, have_prev, x);
 }
 
-void launch_targets(int nc, const double* du, Ref ref, const double* rnorm2, float* t, hipStream_t s) {
-    if (nc > 0) hipLaunchKernelGGL(k_targets, grid_for(nc), dim3(kBlock), 0, s, nc, du, ref, rnorm2, t);
+void launch_targets(int nc, const double* du, const double* du_prev, int have_prev, Ref ref, const double* rnorm2,
+                    float* t, hipStream_t s) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_targets, grid_for(nc), dim3(kBlock), 0, s, nc, du, du_prev, have_prev, ref, rnorm2, t);
 }
 
-void launch_prediction(int nc, const float* y, Ref ref, const double* rnorm2, double* x0, hipStream_t s) {
-    if (nc > 0) hipLaunchKernelGGL(k_prediction, grid_for(nc), dim3(kBlock), 0, s, nc, y, ref, rnorm2, x0);
+void launch_prediction(int nc, const float* y, const double* du_prev, int have_prev, Ref ref, const double* rnorm2,
+                       double* x0, hipStream_t s) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_prediction, grid_for(nc), dim3(kBlock), 0, s, nc, y, du_prev, have_prev, ref, rnorm2, x0);
 }
 
 void launch_gemm_nt(int n, int m, int k, const bf16* A, const bf16* W, float* C, hipStream_t s) {

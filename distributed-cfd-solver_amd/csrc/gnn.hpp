@@ -13,7 +13,7 @@
 //   h1_i   = mean_{e in adj(i)} relu(z1_e)
 //   z2_e   = WP2 h1_i + WQ2 h1_j + G2 g_e + b2
 //   h2_i   = mean_{e in adj(i)} relu(z2_e)
-//   y_i    = Wo h2_i + bo
+//   y_i    = Wo h2_i + bo   (the predicted update is du_prev + y s rnorm)
 // (the EdgeConv form relu(A x_i + B (x_j - x_i)) with WP = A - B, WQ = B).
 // The node products [WP; WQ] x run on MFMA in BF16 with FP32 accumulation;
 // the edge sums, the output layer, the loss and Adam run in FP32.  Every sum
@@ -94,9 +94,13 @@ void launch_explicit_norm(int nc, const double* res, const double* lam, Ref ref,
 //   20    ln(rnorm / previous rnorm)  21  1 when a previous update exists
 void launch_features(int nc, const double* cs, long long ld, const double* res, const double* lam, const double* dt,
                      Ref ref, const double* rnorm2, const double* du_prev, int have_prev, bf16* x, hipStream_t s);
-// targets t = du / (s rnorm) (du AoS, 5 per cell) and the inverse map x0 = y s rnorm
-void launch_targets(int nc, const double* du, Ref ref, const double* rnorm2, float* t, hipStream_t s);
-void launch_prediction(int nc, const float* y, Ref ref, const double* rnorm2, double* x0, hipStream_t s);
+// The network predicts the change of the update (skip connection in FP64):
+// targets t = (du - du_prev) / (s rnorm) and x0 = du_prev + y s rnorm, with
+// du_prev = 0 when have_prev == 0 (du, du_prev, x0 AoS, 5 per cell).
+void launch_targets(int nc, const double* du, const double* du_prev, int have_prev, Ref ref, const double* rnorm2,
+                    float* t, hipStream_t s);
+void launch_prediction(int nc, const float* y, const double* du_prev, int have_prev, Ref ref, const double* rnorm2,
+                       double* x0, hipStream_t s);
 
 // ---- launchers (gnn.hip) ----
 // C[n x m] = A[n x k] W[m x k]^T, bf16 inputs (row-major), fp32 output; k % 32 == 0, m % 16 == 0.

@@ -26,10 +26,12 @@ constexpr int kStoreF64 = 0, kStoreF32 = 1, kStoreBF16 = 2;
 
 // Cell data: structure of arrays in the device (tile) order, kCellPlanes
 // planes of FieldArgs::ld doubles (ld >= cells + boundary ghosts, a multiple of
-// 64): plane k holds quantity k of every cell and ghost.  Consecutive lanes of
-// a wave work on neighbouring cells of one tile, so every plane access of a
-// wave falls in a few cache lines.
-constexpr int kCellPlanes = 29;
+// 64): plane k holds quantity k of every cell and ghost.  Planes are stored in
+// interleaved pairs -- quantity k of cell c at ((k / 2) ld + c) 2 + k % 2 --
+// so that one 16-byte load brings two quantities of a cell.  Consecutive lanes
+// of a wave work on neighbouring cells of one tile, so every load of a wave
+// falls in a few cache lines.
+constexpr int kCellPlanes = 30;  // 29 used
 constexpr int kRQ = 0;     // p, u, v, w, T
 constexpr int kRC = 5;     // centroid x, y, z
 constexpr int kRPhi = 8;   // limiter of the 5 primitives
@@ -42,27 +44,32 @@ constexpr int kRG = 14;    // gradients, (3 v + d)
 // (owner, neighbour) device ids per face, the neighbour of a boundary face
 // being its ghost n_cells + (f - n_inner).  The flux kernel reads a second
 // copy of the same planes laid out in the order of the tiles' face lists.
-constexpr int kFacePlanes = 11;
+constexpr int kFacePlanes = 12;  // 11 used, paired like the cell planes
 constexpr int kFN = 0, kFA = 3, kFC = 4, kFD = 7, kFDinv = 10;
+
+// Position of quantity k of entry i in paired planes of stride ld.
+__host__ __device__ __forceinline__ long long plane_at(int k, long long ld, long long i) {
+    return ((k >> 1) * ld + i) * 2 + (k & 1);
+}
 
 // Read / write views of one cell or face in the planes.
 struct CellRef {
     const double* b;
     long long ld;
     int c;
-    __host__ __device__ __forceinline__ double operator[](int k) const { return b[k * ld + c]; }
+    __host__ __device__ __forceinline__ double operator[](int k) const { return b[plane_at(k, ld, c)]; }
 };
 struct CellRefW {
     double* b;
     long long ld;
     int c;
-    __host__ __device__ __forceinline__ double& operator[](int k) const { return b[k * ld + c]; }
+    __host__ __device__ __forceinline__ double& operator[](int k) const { return b[plane_at(k, ld, c)]; }
 };
 struct FaceRef {
     const double* b;
     long long ld;
     int f;
-    __host__ __device__ __forceinline__ double operator[](int k) const { return b[k * ld + f]; }
+    __host__ __device__ __forceinline__ double operator[](int k) const { return b[plane_at(k, ld, f)]; }
 };
 
 struct MeshArgs {

@@ -290,7 +290,11 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
     for (int sl = threadIdx.x; sl < ns; sl += kTileCells) {
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
-        for (int k = 0; k < 8; ++k) sq[k * kGradSlots + sl] = f.cs[k * f.ld + src];
+        for (int k = 0; k < 8; k += 2) {  // planes 0-7 = q, centroid: four 16-byte pair loads
+            const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(k, f.ld, src));
+            sq[k * kGradSlots + sl] = v.x;
+            sq[(k + 1) * kGradSlots + sl] = v.y;
+        }
     }
     __syncthreads();
     if (static_cast<int>(threadIdx.x) >= nt) return;
@@ -483,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cs, long lo
                                                double* dt, double* alpha) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const double vol = cs[kRVol * ld + c];
+    const double vol = cs[plane_at(kRVol, ld, c)];
     const double inv_cfl = 1.0 / cfl;
     const double l = dmax(lam[c], 1.0e-14);
     dt[c] = cfl * vol / l;
