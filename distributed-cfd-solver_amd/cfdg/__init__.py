@@ -110,6 +110,25 @@ class Precision(C.Structure):
         return p
 
 
+TURB_NONE, TURB_SA = 0, 1
+TURB_FROZEN, TURB_POINT_IMPLICIT = 0, 1
+
+
+class Turbulence(C.Structure):
+    """cfdg_turbulence: Spalart-Allmaras module parameters ([turbulence] of the config)."""
+    _fields_ = [("model", _i32), ("update", _i32), ("nu_inf_ratio", _d), ("max_distance_sweeps", _i32),
+                ("reserved", _i32), ("distance_tolerance", _d), ("rho_inf", _d), ("p_inf", _d)]
+
+    @classmethod
+    def make(cls, rho_inf, p_inf, model=TURB_SA, update=TURB_POINT_IMPLICIT, nu_inf_ratio=3.0,
+             max_distance_sweeps=500, distance_tolerance=1.0e-8):
+        t = cls()
+        t.model, t.update, t.nu_inf_ratio = model, update, nu_inf_ratio
+        t.max_distance_sweeps, t.reserved, t.distance_tolerance = max_distance_sweeps, 0, distance_tolerance
+        t.rho_inf, t.p_inf = rho_inf, p_inf
+        return t
+
+
 # Mesh array fields of MeshView: name -> (dtype, length attribute)
 MESH_ARRAYS = {
     "cell_type": (np.uint8, "n_cells"), "cell_nodes_off": (np.int32, "n_cells+1"), "cell_nodes": (np.int32, None),
@@ -150,6 +169,8 @@ def _load():
         "cfdg_gnn_loss_grad": [vp, _i32, vp, vp, _i32],
         "cfdg_gnn_get_sample": [vp, _i32, vp, vp, vp, vp, vp, vp, vp, vp], "cfdg_gnn_predict": [vp, vp, vp],
         "cfdg_set_warm_start": [vp, vp], "cfdg_warm_start_stats": [vp, vp, vp],
+        "cfdg_set_turbulence": [vp, vp, vp], "cfdg_set_turbulence_state": [vp, vp],
+        "cfdg_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -420,6 +441,26 @@ class Solver:
         u, r = _i64(), _i64()
         self._chk(lib.cfdg_warm_start_stats(self._h, C.byref(u), C.byref(r)))
         return u.value, r.value
+
+    def set_turbulence(self, turb):
+        """turb: Turbulence; computes the wall distance and sets rho nu_tilde to the freestream."""
+        self._chk(lib.cfdg_set_turbulence(self._h, self.mesh._h, C.byref(turb)))
+
+    def set_turbulence_state(self, rho_nu):
+        rho_nu = np.ascontiguousarray(rho_nu, dtype=np.float64)
+        assert rho_nu.shape == (self.n_cells,)
+        self._chk(lib.cfdg_set_turbulence_state(self._h, _ptr(rho_nu)))
+
+    def turbulence(self):
+        """Module fields of the last residual evaluation (mesh order): rho_nu, nu, mut, dist, res,
+        diag (n_cells), grad (3, n_cells), mdot (n_faces)."""
+        n = self.n_cells
+        out = {k: np.empty(n) for k in ("rho_nu", "nu", "mut", "dist", "res", "diag")}
+        out["grad"] = np.empty((3, n))
+        out["mdot"] = np.empty(self.mesh.n_faces)
+        self._chk(lib.cfdg_get_turbulence(self._h, *(_ptr(out[k]) for k in
+                                                     ("rho_nu", "nu", "mut", "dist", "res", "grad", "mdot", "diag"))))
+        return out
 
     def set_precision(self, prec):
         self._chk(lib.cfdg_set_precision(self._h, C.byref(prec)))

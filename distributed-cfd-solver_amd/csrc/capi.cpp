@@ -168,17 +168,22 @@ void eval_residual(cfdg_ctx& c) {
     hip_check(hipEventRecord(c.ev[0], c.stream), "event");
     cfdg::launch_primitives(m, f, c.nargs, c.stream);
     cfdg::launch_ghosts(m, f, c.nargs, c.bcs, c.stream);
+    // module ghosts, then density / nu_tilde / eddy viscosity (solver.hpp:195-204)
+    if (c.sa_on()) cfdg::launch_sa_prepare(m, f, c.nargs, c.sargs(), c.stream);
     hip_check(hipEventRecord(c.ev[1], c.stream), "event");
     const bool need_grad = c.nargs.muscl || c.nargs.viscous;
     if (need_grad) {
         cfdg::launch_gradient_limiter(m, f, c.nargs, c.stream);
         hip_check(hipEventRecord(c.ev[2], c.stream), "event");
         cfdg::launch_ghost_gradients(m, f, c.nargs, c.bcs, c.stream);
+        if (c.sa_on()) cfdg::launch_sa_gradient(m, f, c.nargs, c.sargs(), c.stream);
     } else {
         hip_check(hipEventRecord(c.ev[2], c.stream), "event");
     }
     hip_check(hipEventRecord(c.ev[3], c.stream), "event");
     cfdg::launch_flux(m, f, c.nargs, c.stream);
+    // module face sweep on the recorded mass flux, then its sources (solver.hpp:233-241)
+    if (c.sa_on()) cfdg::launch_sa_residual(m, f, c.nargs, c.sargs(), c.stream);
     hip_check(hipEventRecord(c.ev[4], c.stream), "event");
     c.check_launch();
     c.sync();
@@ -742,6 +747,9 @@ void implicit_step(cfdg_ctx& c, const cfdg_linparams& lp, cfdg_step_result* out)
     double l2[5];
     residual_norms(c, l2);
     cfdg::launch_state_update(c.nc, c.nc, c.x.get(), c.U.get(), c.stream);
+    // module block: copied (implicit_euler.hpp:64-72) or point-implicit, then clamped
+    if (c.sa_on())
+        cfdg::launch_sa_update(c.nc, c.turb.update == CFDG_TURB_POINT_IMPLICIT, c.alpha.get(), c.sargs(), c.stream);
     c.check_launch();
     c.sync();
     c.have_residual = false;
@@ -1104,6 +1112,113 @@ int cfdg_boundary_integrals(cfdg_ctx* c, double* mass, double* energy, double* f
             if (energy) energy[p] = o[4];
             if (force)
                 for (int k = 0; k < 3; ++k) force[3 * p + k] = o[1 + k];
+        }
+    });
+}
+
+int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulence* t) {
+    return guarded(c, [&] {
+        if (!t) throw CfdgError(CFDG_ERR_ARG, "null turbulence parameters");
+        c->have_residual = false;
+        c->have_dt = false;
+        if (t->model == CFDG_TURB_NONE) {
+            c->turb = *t;
+            return;
+        }
+        if (t->model != CFDG_TURB_SA) throw CfdgError(CFDG_ERR_CONFIG, "unsupported turbulence model");
+        if (t->update != CFDG_TURB_FROZEN && t->update != CFDG_TURB_POINT_IMPLICIT)
+            throw CfdgError(CFDG_ERR_CONFIG, "turbulence update must be FROZEN or POINT_IMPLICIT");
+        // config.cpp:175-186
+        if (c->num.flow_model != CFDG_VISCOUS_FLOW)
+            throw CfdgError(CFDG_ERR_CONFIG, "turbulence requires the viscous flow model");
+        if (!(t->nu_inf_ratio > 0.0) || !(t->distance_tolerance > 0.0) || t->max_distance_sweeps < 0 ||
+            !(t->rho_inf > 0.0) || !(t->p_inf > 0.0))
+            throw CfdgError(CFDG_ERR_ARG, "nu_inf_ratio, distance_tolerance, rho_inf, p_inf > 0 and sweeps >= 0 required");
+        if (!mesh) throw CfdgError(CFDG_ERR_ARG, "null mesh");
+        const cfdg::Mesh& m = mesh->m;
+        if (m.n_cells != c->nc || m.n_faces != c->nf || m.n_inner_faces != c->ninner)
+            throw CfdgError(CFDG_ERR_ARG, "mesh differs from the mesh of cfdg_create");
+        // zone treatment per boundary face (spalart_allmaras.hpp:167-181, 447-464)
+        std::vector<char> wall(z(c->npatch), 0);
+        std::vector<int> zone(z(c->npatch), cfdg::kZoneSymmetry);
+        for (int p = 0; p < c->npatch; ++p) {
+            switch (c->bcs.bc[p].type) {
+                case cfdg::kNoSlip:
+                case cfdg::kNoSlipHF: zone[z(p)] = cfdg::kZoneWall; break;
+                case cfdg::kSupIn:
+                case cfdg::kSubIn:
+                case cfdg::kFar: zone[z(p)] = cfdg::kZoneInflow; break;
+                case cfdg::kSupOut:
+                case cfdg::kSubOut: zone[z(p)] = cfdg::kZoneOutflow; break;
+                default: break;
+            }
+            wall[z(p)] = zone[z(p)] == cfdg::kZoneWall;
+        }
+        std::vector<int> kind(z(c->nbf));
+        for (int b = 0; b < c->nbf; ++b) {
+            const int p = m.face_patch[z(c->ninner + b)];
+            kind[z(b)] = (p >= 0 && p < c->npatch) ? zone[z(p)] : cfdg::kZoneSymmetry;
+        }
+        // wall distance over the vertex stencil (solve_wall_distance, spalart_allmaras.hpp:183-189)
+        c->h_sa_dist = cfdg::wall_distance(m, cfdg::cell_cells_by_node(m), wall, t->max_distance_sweeps,
+                                           t->distance_tolerance);
+        std::vector<double> dist_dev(z(c->nc));
+        for (int r = 0; r < c->nc; ++r) dist_dev[z(r)] = c->h_sa_dist[z(c->new2old[z(r)])];
+        // freestream working values (spalart_allmaras.hpp:191-193)
+        const double T_inf = t->p_inf / (t->rho_inf * c->num.gas_constant);
+        c->sa_nu_inf = t->nu_inf_ratio * cfdg::sutherland(T_inf) / t->rho_inf;
+        c->sa_rho_nu_inf = t->rho_inf * c->sa_nu_inf;
+        c->sa_kind.upload(kind, c->stream);
+        c->sa_dist.upload(dist_dev, c->stream);
+        c->sa_rnu.upload(std::vector<double>(z(c->ntot), c->sa_rho_nu_inf), c->stream);
+        c->sa_nu.alloc(z(c->ntot));
+        c->sa_rho.alloc(z(c->ntot));
+        c->sa_mut.alloc(z(c->ntot));
+        c->sa_mut.zero(c->stream);
+        c->sa_g.alloc(3 * z(c->nc));
+        c->sa_res.alloc(z(c->nc));
+        c->sa_diag.alloc(z(c->nc));
+        c->sa_mdot.alloc(z(c->nf));
+        c->sync();
+        c->turb = *t;
+    });
+}
+
+int cfdg_set_turbulence_state(cfdg_ctx* c, const double* rho_nu) {
+    return guarded(c, [&] {
+        if (!c->sa_on()) throw CfdgError(CFDG_ERR_STATE, "no turbulence model set");
+        if (!rho_nu) throw CfdgError(CFDG_ERR_ARG, "null state");
+        std::vector<double> h(z(c->nc));
+        for (int r = 0; r < c->nc; ++r) h[z(r)] = rho_nu[c->new2old[z(r)]];
+        c->h2d(c->sa_rnu.get(), h.data(), h.size());
+        c->sync();
+        c->have_residual = false;
+        c->have_dt = false;
+    });
+}
+
+int cfdg_get_turbulence(cfdg_ctx* c, double* rho_nu, double* nu, double* mut, double* dist, double* res,
+                        double* grad, double* mdot, double* diag) {
+    return guarded(c, [&] {
+        if (!c->sa_on()) throw CfdgError(CFDG_ERR_STATE, "no turbulence model set");
+        const auto cells = [&](const DBuf<double>& b, double* out, long long off) {
+            if (!out) return;
+            std::vector<double> h(z(c->nc));
+            c->d2h(h.data(), b.get() + off, h.size());
+            c->sync();
+            for (int r = 0; r < c->nc; ++r) out[c->new2old[z(r)]] = h[z(r)];
+        };
+        cells(c->sa_rnu, rho_nu, 0);
+        cells(c->sa_nu, nu, 0);
+        cells(c->sa_mut, mut, 0);
+        cells(c->sa_res, res, 0);
+        cells(c->sa_diag, diag, 0);
+        if (dist) std::copy(c->h_sa_dist.begin(), c->h_sa_dist.end(), dist);
+        if (grad)
+            for (int d = 0; d < 3; ++d) cells(c->sa_g, grad + static_cast<long long>(d) * c->nc, static_cast<long long>(d) * c->nc);
+        if (mdot) {
+            c->d2h(mdot, c->sa_mdot.get(), z(c->nf));
+            c->sync();
         }
     });
 }

@@ -135,6 +135,28 @@ struct cfdg_ctx {
     bool features_had_prev = false;     // du_prev was valid for the current features
     bool have_features = false;
     long long warm_used = 0, warm_rejected = 0;
+    // Spalart-Allmaras module (off unless turb.model == CFDG_TURB_SA)
+    cfdg_turbulence turb{};
+    double sa_nu_inf = 0.0, sa_rho_nu_inf = 0.0;
+    DBuf<int> sa_kind;                                  // zone of each boundary face
+    DBuf<double> sa_dist, sa_rnu, sa_nu, sa_rho, sa_mut, sa_g, sa_res, sa_diag, sa_mdot;
+    std::vector<double> h_sa_dist;                      // wall distance, mesh order
+    bool sa_on() const { return turb.model == CFDG_TURB_SA; }
+    cfdg::SaArgs sargs() const {
+        cfdg::SaArgs a{};
+        a.nu_inf = sa_nu_inf;
+        a.kind = sa_kind.get();
+        a.dist = sa_dist.get();
+        a.rnu = sa_rnu.get();
+        a.nu = sa_nu.get();
+        a.rho = sa_rho.get();
+        a.mut = sa_mut.get();
+        a.g = sa_g.get();
+        a.res = sa_res.get();
+        a.diag = sa_diag.get();
+        a.mdot = sa_mdot.get();
+        return a;
+    }
     // timing
     hipEvent_t ev[8] = {};
     double stage_ms[6] = {0, 0, 0, 0, 0, 0};
@@ -176,6 +198,8 @@ struct cfdg_ctx {
         f.res = res.get();
         f.lam = lam.get();
         f.bflux = bflux.get();
+        f.mut = sa_on() ? sa_mut.get() : nullptr;
+        f.mdot = sa_on() ? sa_mdot.get() : nullptr;
         return f;
     }
     cfdg::BsrArgs bargs() const {

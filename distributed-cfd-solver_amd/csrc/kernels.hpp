@@ -108,6 +108,27 @@ struct FieldArgs {
     double* res;      // 5 planes of nc
     double* lam;      // nc
     double* bflux;    // F (inviscid + viscous) of every boundary face, 5 per face
+    const double* mut;  // eddy viscosity (cells + ghosts, ghosts zero), null without turbulence
+    double* mdot;       // mass flux F[0] of every face (mesh order), written when mut is set
+};
+
+// Zone treatment of nu_tilde at a boundary face (spalart_allmaras.hpp:54-59)
+constexpr int kZoneWall = 0, kZoneSymmetry = 1, kZoneInflow = 2, kZoneOutflow = 3;
+
+// Spalart-Allmaras module fields, device order; arrays of ntot = cells +
+// boundary ghosts hold the ghost of boundary face f at n_cells + (f - n_inner).
+struct SaArgs {
+    double nu_inf;        // freestream nu_tilde
+    const int* kind;      // zone of each boundary face
+    const double* dist;   // wall distance (nc)
+    double* rnu;          // rho nu_tilde (ntot): cells = state, ghosts = zone values
+    double* nu;           // nu_tilde (ntot)
+    double* rho;          // density (ntot)
+    double* mut;          // eddy viscosity (ntot, ghosts zero)
+    double* g;            // nu_tilde gradient, 3 planes of nc
+    double* res;          // module residual (nc)
+    double* diag;         // point-implicit diagonal (nc)
+    const double* mdot;   // mean-flow face mass flux (nf, mesh order)
 };
 
 struct NumArgs {
@@ -153,5 +174,12 @@ void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double 
 // values: nnzb x 25 entries of the storage type (kStoreF64 / kStoreF32 / kStoreBF16)
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
                      const double* alpha, void* values, int storage, hipStream_t s);
+
+// --- Spalart-Allmaras module (kernels_sa.hip) --------------------------------
+void launch_sa_prepare(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s);
+void launch_sa_gradient(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s);
+// res / diag of the module and lam += the interior faces' module spectral radius
+void launch_sa_residual(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s);
+void launch_sa_update(int nc, int point_implicit, const double* alpha, const SaArgs& sa, hipStream_t s);
 
 }  // namespace cfdg

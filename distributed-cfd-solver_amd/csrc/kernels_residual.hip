@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
 // contribution (residual_kernel.hpp:183-240 / 245-300 for one face, with the
 // MUSCL states of muscl.hpp:108-157).
 // fr views the face's planes, (c0, c1) = (owner, neighbour or ghost).
-template <class FR>
+template <bool kTurb, class FR>
 __device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm, const FR& fr, bool inner, int c0,
                                           int c1, double F[5], double& lam_f) {
     const CellRef r0 = crec(f, c0);
@@ -436,8 +436,11 @@ __device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm,
         }
         double Fv[5];
         double lv = 0.0;
-        viscous_flux(nm.gas, fr[kFDinv], fr[kFD], fr[kFD + 1], fr[kFD + 2], qa, qb, ga, gb, 0.0, 0.0, nx, ny, nz, A, Fv,
-                     lv);
+        // eddy viscosity of both sides (residual_kernel.hpp:229-230, 282-283)
+        const double mutL = kTurb ? f.mut[c0] : 0.0;
+        const double mutR = kTurb ? f.mut[c1] : 0.0;
+        viscous_flux(nm.gas, fr[kFDinv], fr[kFD], fr[kFD + 1], fr[kFD + 2], qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A,
+                     Fv, lv);
         for (int v = 1; v < 5; ++v) F[v] += Fv[v];
         lam_f += lv;
     }
@@ -447,7 +450,9 @@ __device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm,
 // parallel) into LDS, then each cell adds its faces' fluxes in ascending face
 // index, with + for the owner and - for the neighbour (residual_kernel.hpp:
 // 175-306 formed as a gather; only faces on a tile boundary are evaluated by
-// both adjacent tiles).
+// both adjacent tiles).  With turbulence the kernel also records each face's
+// mass flux F[0] (from the tile that owns the face's owner cell).
+template <bool kTurb>
 __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
     __shared__ double sF[6][kTileFaces];
     const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -456,7 +461,8 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
         double F[5], lam_f;
         const int fc = m.tile_faces[fb + i];
         const int2 cc = m.tcells[fb + i];
-        face_flux(f, nm, FaceRef{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
+        face_flux<kTurb>(f, nm, FaceRef{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
+        if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
         for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
         sF[5][i] = lam_f;
         if (fc >= m.ninner)  // a boundary face lies in exactly one tile: single writer
@@ -531,7 +537,10 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, Nu
         const double half_area = 0.5 * A;
         const double diss = 0.5 * smax * A;
         double vm[9];
-        if (nm.viscous) viscous_jacobian_entries(nm.gas, fr[kFDinv], qL, qR, 0.0, 0.0, A, vm);
+        // jacobian_kernel.hpp:156-157, 209-210, 252-254: the ghost side carries no eddy viscosity
+        const double mutL = f.mut ? f.mut[c0] : 0.0;
+        const double mutR = (f.mut && inner) ? f.mut[c1] : 0.0;
+        if (nm.viscous) viscous_jacobian_entries(nm.gas, fr[kFDinv], qL, qR, mutL, mutR, A, vm);
         double J[25], Jv[25];
         const bool own = (c0 == c);
         // dFL = 0.5 A J(UL) + diss I (+ viscous), dFR = 0.5 A J(UR) - diss I (- viscous);
@@ -620,7 +629,11 @@ void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs
 }
 
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s) {
-    if (m.ntiles > 0) hipLaunchKernelGGL(k_flux, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
+    if (m.ntiles <= 0) return;
+    if (f.mut)
+        hipLaunchKernelGGL(k_flux<true>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
+    else
+        hipLaunchKernelGGL(k_flux<false>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
 }
 
 void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patches, double* out, hipStream_t s) {

@@ -1,0 +1,297 @@
+// Spalart-Allmaras module kernels (gfx950), device cell order.
+//
+//   k_sa_prepare   density, nu_tilde and eddy viscosity of the cells; zone
+//                  ghost values of the boundary faces (one thread per cell or
+//                  boundary face)
+//   k_sa_gradient  nu_tilde gradient with the configured method, one thread
+//                  per cell (node least squares rebuilt from the centroids as
+//                  the mean-flow gradient kernel does, or Green-Gauss)
+//   k_sa_residual  per cell, over its faces in ascending face index: upwind
+//                  convection on the recorded mean mass flux, skew-corrected
+//                  diffusion, the interior faces' spectral radius added to
+//                  lam; then the production / destruction / cb2 sources and
+//                  the point-implicit diagonal
+//   k_sa_update    frozen (positivity clamp only) or point-implicit update
+//
+// The equations follow spalart_allmaras.hpp:196-442 with four corrections:
+// the diffusive flux leaves the owner as -coef (grad nu . n) A, cb2 is a
+// source, the freestream override of a boundary face with inflowing mass is
+// limited to inflow / outflow zones, and fw has the (g^6 + cw3^6) denominator
+// of the header's formula (spalart_allmaras.hpp:17).  Single writer per output, fixed
+// summation order, no atomics.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace cfdg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kCb1 = 0.1355, kCb2 = 0.622, kSigma = 2.0 / 3.0, kKappa = 0.41;
+constexpr double kCw2 = 0.3, kCw3 = 2.0, kCv1 = 7.1;
+constexpr double kCw1 = kCb1 / (kKappa * kKappa) + (1.0 + kCb2) / kSigma;
+constexpr double kCv1_3 = kCv1 * kCv1 * kCv1;
+constexpr double kSbarFloor = 1.0e-10, kMinWallDist = 1.0e-10;
+
+__device__ __forceinline__ CellRef crec(const FieldArgs& f, int c) { return CellRef{f.cs, f.ld, c}; }
+__device__ __forceinline__ FaceRef frec(const MeshArgs& m, int f) { return FaceRef{m.fp, m.fld, f}; }
+
+__device__ __forceinline__ double fv1_of(double chi) {
+    const double chi3 = chi * chi * chi;
+    return chi3 / (chi3 + kCv1_3);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_prepare(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nbf = m.nf - m.ninner;
+    if (i >= m.nc + nbf) return;
+    const CellRef r = crec(f, i);
+    const double rho = r[kRQ] / (nm.gas.R * r[kRQ + 4]);  // density_Tp
+    if (i < m.nc) {
+        const double nu = sa.rnu[i] / rho;
+        sa.rho[i] = rho;
+        sa.nu[i] = nu;
+        const double nu_lam = sutherland(r[kRQ + 4]) / rho;
+        sa.mut[i] = rho * nu * fv1_of(nu / nu_lam);
+        return;
+    }
+    const int b = i - m.nc;  // boundary face ninner + b, ghost slot i
+    const int c0 = m.fcells[m.ninner + b].x;
+    double rg;
+    switch (sa.kind[b]) {
+        case kZoneWall: rg = -sa.rnu[c0]; break;
+        case kZoneInflow: rg = rho * sa.nu_inf; break;
+        default: rg = sa.rnu[c0]; break;
+    }
+    sa.rnu[i] = rg;
+    sa.rho[i] = rho;
+    sa.nu[i] = rg / rho;
+    sa.mut[i] = 0.0;  // BC ghosts carry no turbulence (spalart_allmaras.hpp:266-268)
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_gradient(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    const CellRef rc = crec(f, c);
+    const double s0 = sa.nu[c];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (nm.gradient == 2) {
+        const double x0 = rc[kRC], y0 = rc[kRC + 1], z0 = rc[kRC + 2];
+        const int jb = m.ccn_off[c], je = m.ccn_off[c + 1];
+        if (je > jb) {
+            double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
+            for (int j = jb; j < je; ++j) {
+                const CellRef rn = crec(f, m.ccn[j]);
+                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+                const double w = 1.0 / dmax(dx * dx + dy * dy + dz * dz, 1.0e-30);
+                A0 += w * dx * dx;
+                A1 += w * dx * dy;
+                A2 += w * dx * dz;
+                A4 += w * dy * dy;
+                A5 += w * dy * dz;
+                A8 += w * dz * dz;
+            }
+            const double A3 = A1, A6 = A2, A7 = A5;
+            A0 += 1.0e-14;
+            A4 += 1.0e-14;
+            A8 += 1.0e-14;
+            double inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const double det = A0 * (A4 * A8 - A5 * A7) - A1 * (A3 * A8 - A5 * A6) + A2 * (A3 * A7 - A4 * A6);
+            if (!(fabs(det) <= 1.0e-14)) {
+                const double id = 1.0 / det;
+                inv[0] = (A4 * A8 - A5 * A7) * id;
+                inv[1] = (A2 * A7 - A1 * A8) * id;
+                inv[2] = (A1 * A5 - A2 * A4) * id;
+                inv[3] = (A5 * A6 - A3 * A8) * id;
+                inv[4] = (A0 * A8 - A2 * A6) * id;
+                inv[5] = (A2 * A3 - A0 * A5) * id;
+                inv[6] = (A3 * A7 - A4 * A6) * id;
+                inv[7] = (A1 * A6 - A0 * A7) * id;
+                inv[8] = (A0 * A4 - A1 * A3) * id;
+            }
+            for (int j = jb; j < je; ++j) {
+                const int n = m.ccn[j];
+                const CellRef rn = crec(f, n);
+                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+                const double w = 1.0 / dmax(dx * dx + dy * dy + dz * dz, 1.0e-30);
+                const double rx = w * dx, ry = w * dy, rz = w * dz;
+                const double d = sa.nu[n] - s0;
+                g0 += (inv[0] * rx + inv[1] * ry + inv[2] * rz) * d;
+                g1 += (inv[3] * rx + inv[4] * ry + inv[5] * rz) * d;
+                g2 += (inv[6] * rx + inv[7] * ry + inv[8] * rz) * d;
+            }
+        }
+    } else {
+        for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
+            const int fc = m.cf[e];
+            const FaceRef fr = frec(m, fc);
+            const int2 oc = m.fcells[fc];
+            const double a05 = 0.5 * fr[kFA];
+            const double s = sa.nu[oc.x] + sa.nu[oc.y];
+            const double fx = a05 * fr[kFN] * s, fy = a05 * fr[kFN + 1] * s, fz = a05 * fr[kFN + 2] * s;
+            if (oc.x == c) {
+                g0 += fx;
+                g1 += fy;
+                g2 += fz;
+            } else {
+                g0 -= fx;
+                g1 -= fy;
+                g2 -= fz;
+            }
+        }
+        const double iv = 1.0 / rc[kRVol];
+        g0 *= iv;
+        g1 *= iv;
+        g2 *= iv;
+    }
+    sa.g[c] = g0;
+    sa.g[static_cast<long long>(m.nc) + c] = g1;
+    sa.g[2LL * m.nc + c] = g2;
+}
+
+// nu_tilde gradient of a cell or (mirrored from its owner) of a ghost
+__device__ __forceinline__ void grad_of(const MeshArgs& m, const SaArgs& sa, int c, double g[3]) {
+    int src = c;
+    double s = 1.0;
+    if (c >= m.nc) {
+        const int b = c - m.nc;
+        src = m.fcells[m.ninner + b].x;
+        s = (sa.kind[b] == kZoneWall) ? -1.0 : 1.0;
+    }
+    g[0] = s * sa.g[src];
+    g[1] = s * sa.g[static_cast<long long>(m.nc) + src];
+    g[2] = s * sa.g[2LL * m.nc + src];
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_residual(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
+    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    constexpr double inv_sigma = 1.0 / kSigma;
+    const CellRef rc = crec(f, c);
+    const double rho_c = sa.rho[c];
+    double res = 0.0, diag = 0.0;
+    double lam = f.lam[c];
+    for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
+        const int fc = m.cf[e];
+        const FaceRef fr = frec(m, fc);
+        const int2 oc = m.fcells[fc];
+        const bool inner = fc < m.ninner;
+        const double mf = sa.mdot[fc];
+        const double nu0 = sa.nu[oc.x], nu1_cell = sa.nu[oc.y];
+        const CellRef r0 = crec(f, oc.x), r1 = crec(f, oc.y);
+        double nu_up, nu1;
+        if (inner) {
+            nu_up = (mf >= 0.0) ? nu0 : nu1_cell;
+            nu1 = nu1_cell;
+        } else {
+            const int kind = sa.kind[fc - m.ninner];
+            const bool inflow = mf < 0.0 && (kind == kZoneInflow || kind == kZoneOutflow);
+            nu_up = inflow ? sa.nu_inf : nu0;
+            nu1 = inflow ? sa.nu_inf : sa.rnu[oc.y] / sa.rho[oc.y];
+        }
+        const double f_conv = mf * nu_up;
+        // averaged + skew-corrected face gradient (spalart_allmaras.hpp:500-525)
+        double ga[3], gb[3];
+        grad_of(m, sa, oc.x, ga);
+        grad_of(m, sa, oc.y, gb);
+        const double dx = fr[kFD], dy = fr[kFD + 1], dz = fr[kFD + 2], inv_d = fr[kFDinv];
+        double gn0 = 0.5 * (ga[0] + gb[0]), gn1 = 0.5 * (ga[1] + gb[1]), gn2 = 0.5 * (ga[2] + gb[2]);
+        const double jump = nu1 - nu0;
+        const double gd = gn0 * dx + gn1 * dy + gn2 * dz;
+        const double corr = (jump - gd) * inv_d * inv_d;
+        gn0 += corr * dx;
+        gn1 += corr * dy;
+        gn2 += corr * dz;
+        const double Tf = 0.5 * (r0[kRQ + 4] + r1[kRQ + 4]);
+        const double mu_f = sutherland(Tf);
+        const double rho_f = 0.5 * (sa.rho[oc.x] + sa.rho[oc.y]);
+        const double A = fr[kFA];
+        const double coef = inner ? inv_sigma * (mu_f + rho_f * (0.5 * (nu0 + nu1)))
+                                  : inv_sigma * (mu_f + rho_f * 0.5 * (nu0 + nu1));
+        const double f_diff = coef * (gn0 * fr[kFN] + gn1 * fr[kFN + 1] + gn2 * fr[kFN + 2]) * A;
+        const double flux = f_conv - f_diff;
+        const double dcoef = coef * A * inv_d;
+        if (inner) {
+            const double visc_term = 2.0 * inv_sigma * (mu_f + rho_f * (0.5 * (nu0 + nu1))) / rho_f * A * inv_d;
+            lam += fabs(mf) / rho_f + visc_term;
+            if (oc.x == c) {
+                res += flux;
+                diag += (dmax(mf, 0.0) + dcoef) / rho_c;
+            } else {
+                res -= flux;
+                diag += (dmax(-mf, 0.0) + dcoef) / rho_c;
+            }
+        } else {
+            res += flux;
+            diag += (dmax(mf, 0.0) + dcoef) / rho_c;
+        }
+    }
+    // sources (spalart_allmaras.hpp:382-432)
+    const double nu = sa.nu[c];
+    const double d = dmax(sa.dist[c], kMinWallDist);
+    const double inv_d2 = 1.0 / (d * d);
+    const double inv_k2 = 1.0 / (kKappa * kKappa);
+    const double ux = rc[kRG + 3], uy = rc[kRG + 4], uz = rc[kRG + 5];
+    const double vx = rc[kRG + 6], vy = rc[kRG + 7], vz = rc[kRG + 8];
+    const double wx = rc[kRG + 9], wy = rc[kRG + 10], wz = rc[kRG + 11];
+    const double s_xy = uy + vx, s_xz = uz + wx, s_yz = vz + wy;
+    const double s2 = 2.0 * (ux * ux + vy * vy + wz * wz) + 0.5 * (s_xy * s_xy + s_xz * s_xz + s_yz * s_yz);
+    const double S = sqrt(dmax(s2, 0.0));
+    const double nu_lam = sutherland(rc[kRQ + 4]) / rho_c;
+    const double chi = nu / nu_lam;
+    const double fv1 = fv1_of(chi);
+    const double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    const double sbar = dmax(S + nu * fv2 * inv_k2 * inv_d2, kSbarFloor);
+    double r = nu / (sbar * kKappa * kKappa * d * d);
+    r = dmin(dmax(r, 0.0), 10.0);
+    const double r2 = r * r;
+    const double r6 = r2 * r2 * r2;
+    const double g = r + kCw2 * (r6 - r);
+    const double g2 = dmax(g * g, kSbarFloor);
+    const double g6 = g2 * g2 * g2;
+    constexpr double cw3_6 = kCw3 * kCw3 * kCw3 * kCw3 * kCw3 * kCw3;
+    const double fw = g * sqrt(cbrt((1.0 + cw3_6) / (g6 + cw3_6)));
+    const double P = kCb1 * sbar * rho_c * nu;
+    const double D = kCw1 * fw * rho_c * nu * nu * inv_d2;
+    const double gx = sa.g[c], gy = sa.g[static_cast<long long>(m.nc) + c], gz = sa.g[2LL * m.nc + c];
+    const double cb2 = (kCb2 / kSigma) * rho_c * (gx * gx + gy * gy + gz * gz);
+    const double vol = rc[kRVol];
+    res += vol * (D - P - cb2);
+    const double lin = 2.0 * kCw1 * fw * nu / (d * d) - kCb1 * sbar;
+    diag += vol * dmax(lin, 0.0);
+    sa.res[c] = res;
+    sa.diag[c] = diag;
+    f.lam[c] = lam;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_update(int nc, int point_implicit, const double* alpha, SaArgs sa) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double v = sa.rnu[c];
+    if (point_implicit) v = v - sa.res[c] / (alpha[c] + sa.diag[c]);
+    sa.rnu[c] = (v < 0.0) ? 0.0 : v;
+}
+
+int blocks(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_sa_prepare(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s) {
+    const long long n = static_cast<long long>(m.nc) + (m.nf - m.ninner);
+    k_sa_prepare<<<blocks(n), kBlock, 0, s>>>(m, f, nm, sa);
+}
+
+void launch_sa_gradient(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s) {
+    k_sa_gradient<<<blocks(m.nc), kBlock, 0, s>>>(m, f, nm, sa);
+}
+
+void launch_sa_residual(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const SaArgs& sa, hipStream_t s) {
+    k_sa_residual<<<blocks(m.nc), kBlock, 0, s>>>(m, f, nm, sa);
+}
+
+void launch_sa_update(int nc, int point_implicit, const double* alpha, const SaArgs& sa, hipStream_t s) {
+    k_sa_update<<<blocks(nc), kBlock, 0, s>>>(nc, point_implicit, alpha, sa);
+}
+
+}  // namespace cfdg

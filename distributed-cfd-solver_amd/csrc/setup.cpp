@@ -414,4 +414,51 @@ std::vector<i32> backward_levels(const Csr& ccf, i32 n, i32& n_levels) {
     return lev;
 }
 
+// Wall distance by graph relaxation (aux_geometry.cpp:545-697, one rank):
+// cells next to a wall face start at |x_f - x_c| (minimum over their wall
+// faces), all others at 1e30; each sweep relaxes d_c = min(d_c, d_n + |x_n - x_c|)
+// over the stencil in ascending and then descending cell order (Gauss-Seidel,
+// updated values used at once) until the largest relative decrease of a sweep
+// is at most rel_tol or max_sweeps sweeps have run.
+std::vector<double> wall_distance(const Mesh& m, const Csr& st, const std::vector<char>& wall_patch, int max_sweeps,
+                                  double rel_tol) {
+    constexpr double kInf = 1.0e30, kMinDist = 1.0e-12;
+    std::vector<double> d(z(m.n_cells), kInf);
+    for (i32 f = m.n_inner_faces; f < m.n_faces; ++f) {
+        const i32 p = m.face_patch[z(f)];
+        if (p < 0 || z(p) >= wall_patch.size() || !wall_patch[z(p)]) continue;
+        const std::size_t c = z(m.face_owner[z(f)]);
+        const double ex = m.face_cx[z(f)] - m.cell_cx[c], ey = m.face_cy[z(f)] - m.cell_cy[c],
+                     ez = m.face_cz[z(f)] - m.cell_cz[c];
+        d[c] = std::min(d[c], std::sqrt(ex * ex + ey * ey + ez * ez));
+    }
+    std::vector<double> len(st.idx.size());
+    for (i32 c = 0; c < m.n_cells; ++c)
+        for (i32 j = st.off[z(c)]; j < st.off[z(c) + 1]; ++j) {
+            const std::size_t n = z(st.idx[z(j)]);
+            const double dx = m.cell_cx[n] - m.cell_cx[z(c)], dy = m.cell_cy[n] - m.cell_cy[z(c)],
+                         dz = m.cell_cz[n] - m.cell_cz[z(c)];
+            len[z(j)] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        double worst = 0.0;
+        const auto relax = [&](std::size_t c) {
+            const double d0 = d[c];
+            double dn = d0;
+            for (i32 j = st.off[c]; j < st.off[c + 1]; ++j) {
+                const double cand = d[z(st.idx[z(j)])] + len[z(j)];
+                if (cand < dn) dn = cand;
+            }
+            if (dn < d0) {
+                worst = std::max(worst, (d0 - dn) / std::max(d0, kMinDist));
+                d[c] = dn;
+            }
+        };
+        for (std::size_t c = 0; c < z(m.n_cells); ++c) relax(c);
+        for (std::size_t c = z(m.n_cells); c-- > 0;) relax(c);
+        if (worst <= rel_tol) break;
+    }
+    return d;
+}
+
 }  // namespace cfdg

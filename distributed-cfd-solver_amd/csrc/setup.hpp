@@ -31,6 +31,11 @@ std::vector<double> lsq_coefficients(const Mesh& m, const Csr& stencil);
 void face_distances(const Mesh& m, std::vector<double>& dx, std::vector<double>& dy, std::vector<double>& dz,
                     std::vector<double>& dinv);
 
+// Distance of every cell centroid to the nearest wall face centroid along the
+// stencil graph (aux_geometry.cpp:545-697); wall_patch[p] marks the wall patches.
+std::vector<double> wall_distance(const Mesh& m, const Csr& stencil, const std::vector<char>& wall_patch,
+                                  int max_sweeps, double rel_tol);
+
 struct BsrPattern {
     std::vector<std::int32_t> row_ptr, cols, diag;
 };

@@ -309,6 +309,59 @@ typedef struct {
 int cfdg_set_precision(cfdg_ctx* ctx, const cfdg_precision* prec);
 int cfdg_get_precision(cfdg_ctx* ctx, cfdg_precision* prec);
 
+/* ---- turbulence module ----------------------------------------------------
+   Replaces the reference's Spalart-Allmaras physics module (SpalartAllmaras,
+   spalart_allmaras.hpp:61-544; config [turbulence] model / nu_inf_ratio /
+   max_distance_sweeps / distance_tolerance, config.cpp:155-189), which does
+   not compile there.  One transported variable rho nu_tilde per cell.  Each
+   residual evaluation adds, in the reference's order (solver.hpp:195-241):
+   zone ghosts (no-slip wall: -inner; inlet / farfield: rho_g nu_inf;
+   outlet, slip wall, symmetry: inner), density / nu_tilde / eddy viscosity
+   mu_t = rho nu_tilde fv1 (ghosts 0), the nu_tilde gradient with the
+   configured method, mu_t in the mean viscous fluxes and Jacobian blocks, the
+   face mass flux F[0] recorded by the flux sweep, upwind convection and
+   skew-corrected diffusion (interior faces add their spectral radius to
+   lambda) and the production / destruction / cb2 sources.  Four corrections
+   of the reference's code: the diffusive flux leaves the owner as
+   -coef (grad nu . n) A, the cb2 term is a source, the freestream override
+   of a boundary face with inflowing mass applies to inflow / outflow zones
+   only, and fw = g ((1 + cw3^6) / (g^6 + cw3^6))^(1/6) as the header's
+   formula states (its code drops the cw3^6 of the denominator).
+   Update after each implicit step: CFDG_TURB_FROZEN copies the block
+   unchanged as the reference's implicit Euler does (implicit_euler.hpp:64-72);
+   CFDG_TURB_POINT_IMPLICIT sets rho nu_tilde -= res / (V / dt + D) with D the
+   diagonal of the module's upwind convection, diffusion and destruction
+   linearisation.  Both end with the positivity clamp rho nu_tilde >= 0
+   (spalart_allmaras.hpp:435-442). */
+#define CFDG_TURB_NONE 0
+#define CFDG_TURB_SA   1
+#define CFDG_TURB_FROZEN         0
+#define CFDG_TURB_POINT_IMPLICIT 1
+
+typedef struct {
+    int32_t model;               /* CFDG_TURB_NONE | CFDG_TURB_SA (needs the viscous flow model) */
+    int32_t update;              /* CFDG_TURB_FROZEN | CFDG_TURB_POINT_IMPLICIT */
+    double nu_inf_ratio;         /* freestream nu_tilde / nu (reference default 3) */
+    int32_t max_distance_sweeps; /* wall-distance relaxation sweeps (500) */
+    int32_t reserved;            /* 0 */
+    double distance_tolerance;   /* wall-distance relative tolerance (1e-8) */
+    double rho_inf, p_inf;       /* freestream density and pressure ([initial]) */
+} cfdg_turbulence;
+
+/* SpalartAllmaras::initialize + init_state (spalart_allmaras.hpp:157-193):
+   zone kinds from the patches' conditions, wall distance from the no-slip
+   walls over the vertex stencil, nu_inf = ratio mu(T_inf) / rho_inf, every
+   cell at rho_inf nu_inf.  mesh must be the mesh of cfdg_create. */
+int cfdg_set_turbulence(cfdg_ctx* ctx, const cfdg_mesh* mesh, const cfdg_turbulence* turb);
+/* rho nu_tilde of every cell (mesh order) */
+int cfdg_set_turbulence_state(cfdg_ctx* ctx, const double* rho_nu);
+/* Fields of the module after the last residual evaluation, mesh order, each
+   argument optional: rho_nu, nu, mut, dist, res (n_cells), grad
+   (3 x n_cells), mdot (n_faces, face mass flux), diag (point-implicit
+   diagonal, n_cells). */
+int cfdg_get_turbulence(cfdg_ctx* ctx, double* rho_nu, double* nu, double* mut, double* dist, double* res,
+                        double* grad, double* mdot, double* diag);
+
 /* ---- learned warm start ---------------------------------------------------
    No counterpart in the reference (its BiCGSTAB always starts from x = 0,
    implicit_euler.hpp:81-92; the solver API takes the guess in x,

@@ -414,6 +414,14 @@ struct oracle_ctx {
     std::vector<i32> row_ptr, cols, diag;
     std::vector<double> values;
     std::vector<double> du;
+    // Spalart-Allmaras module (spalart_allmaras.hpp:61-544); off when model == 0
+    struct {
+        oracle_turbulence cfg{};
+        double nu_inf = 0.0, rho_nu_inf = 0.0;
+        std::vector<int> kind;  // zone treatment per boundary face (NuZoneKind)
+        std::vector<double> dist, rnu, nu, rho, mut, g, res, mdot, diag;
+    } sa;
+    bool sa_on() const { return sa.cfg.model == 1; }
 };
 
 namespace {
@@ -1043,11 +1051,15 @@ void compute_fluxes(oracle_ctx& c) {
         double F[5];
         double smax = 0.0;
         hllc(c.eos, UL, UR, nxf, nyf, nzf, Af, F, smax);
+        if (c.sa_on()) c.sa.mdot[fs] = F[0];  // residual_kernel.hpp:222-224, 275-277
         double lam_f = Af * smax;
         if (visc) {
             double Fv[5] = {0, 0, 0, 0, 0};
             double lv = 0.0;
-            viscous_flux(c.eos, c.dinv[fs], c.dvx[fs], c.dvy[fs], c.dvz[fs], c0, c1, vv, 0.0, 0.0, nxf, nyf, nzf, Af, Fv, lv,
+            // eddy viscosity of the two sides, zero in BC ghosts (residual_kernel.hpp:229-230, 282-283)
+            const double mutL = c.sa_on() ? c.sa.mut[c0] : 0.0;
+            const double mutR = c.sa_on() ? c.sa.mut[c1] : 0.0;
+            viscous_flux(c.eos, c.dinv[fs], c.dvx[fs], c.dvy[fs], c.dvz[fs], c0, c1, vv, mutL, mutR, nxf, nyf, nzf, Af, Fv, lv,
                          c.num.prandtl, c.num.prandtl_turb);
             for (int v = 1; v < 5; ++v) F[v] += Fv[v];
             lam_f += lv;
@@ -1065,6 +1077,321 @@ void compute_fluxes(oracle_ctx& c) {
     }
 }
 
+// --- Spalart-Allmaras module (spalart_allmaras.hpp) --------------------------
+// The reference header does not compile (it includes headers that do not exist
+// and is never dispatched), so this restatement follows its equations and
+// call order with three corrections, each marked "correction":
+//   1. the diffusive face flux leaves the owner as -coef (grad nu . n) A
+//      (the header adds +coef (...) A, an anti-diffusion);
+//   2. the cb2 term is a source, res -= V cb2/sigma rho |grad nu|^2, as the
+//      header's own comment states (its code adds it with the sign of D);
+//   3. the "mass flux < 0 -> freestream" override of a boundary face applies
+//      to inflow / outflow zones only; at walls and symmetry planes the zone
+//      ghost is used whatever the sign of a round-off mass flux;
+//   4. fw = g ((1 + cw3^6) / (g^6 + cw3^6))^(1/6) as the header's own formula
+//      (spalart_allmaras.hpp:17) states; its code drops the cw3^6 of the
+//      denominator (spalart_allmaras.hpp:419), which gives fw(r = 1) = 2.005.
+// BC ghosts of nu_tilde carry rnu_g / rho_g for the Green-Gauss gradients
+// (the header leaves them zero; the node LSQ stencil does not read them).
+constexpr double kSaCb1 = 0.1355, kSaCb2 = 0.622, kSaSigma = 2.0 / 3.0, kSaKappa = 0.41;
+constexpr double kSaCw2 = 0.3, kSaCw3 = 2.0, kSaCv1 = 7.1;
+constexpr double kSaCw1 = kSaCb1 / (kSaKappa * kSaKappa) + (1.0 + kSaCb2) / kSaSigma;
+constexpr double kSaCv1_3 = kSaCv1 * kSaCv1 * kSaCv1;
+constexpr double kSaSbarFloor = 1.0e-10, kSaMinWallDist = 1.0e-10;
+enum NuZone { ZWall = 0, ZSymmetry, ZInflow, ZOutflow };
+void wall_distance(oracle_ctx& c, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist);
+
+int nu_zone(int bc_type) {  // spalart_allmaras.hpp:447-464
+    switch (bc_type) {
+        case NoSlip:
+        case NoSlipHF: return ZWall;
+        case SupIn:
+        case SubIn:
+        case Far: return ZInflow;
+        case SupOut:
+        case SubOut: return ZOutflow;
+        default: return ZSymmetry;
+    }
+}
+
+double sa_fv1(double chi) {
+    const double chi3 = chi * chi * chi;
+    return chi3 / (chi3 + kSaCv1_3);
+}
+
+// cell_sources (spalart_allmaras.hpp:382-432) for one cell: P, D, the cb2
+// term and the pieces of the point-implicit diagonal
+struct SaSource {
+    double P, D, cb2, fv1, fv2, sbar, r, fw;
+};
+SaSource sa_source(double rho, double nu, double mu_lam, double d_in, double S, double grad2) {
+    SaSource o{};
+    const double inv_k2 = 1.0 / (kSaKappa * kSaKappa);
+    const double d = std::max(d_in, kSaMinWallDist);
+    const double inv_d2 = 1.0 / (d * d);
+    const double nu_lam = mu_lam / rho;
+    const double chi = nu / nu_lam;
+    o.fv1 = sa_fv1(chi);
+    o.fv2 = 1.0 - chi / (1.0 + chi * o.fv1);
+    o.sbar = std::max(S + nu * o.fv2 * inv_k2 * inv_d2, kSaSbarFloor);
+    double r = nu / (o.sbar * kSaKappa * kSaKappa * d * d);
+    r = std::min(std::max(r, 0.0), 10.0);
+    o.r = r;
+    const double r2 = r * r;
+    const double r6 = r2 * r2 * r2;
+    const double g = r + kSaCw2 * (r6 - r);
+    const double g2 = std::max(g * g, kSaSbarFloor);
+    const double g6 = g2 * g2 * g2;
+    const double cw3_6 = kSaCw3 * kSaCw3 * kSaCw3 * kSaCw3 * kSaCw3 * kSaCw3;
+    o.fw = g * std::sqrt(std::cbrt((1.0 + cw3_6) / (g6 + cw3_6)));  // correction 4
+    o.P = kSaCb1 * o.sbar * rho * nu;
+    o.D = kSaCw1 * o.fw * rho * nu * nu * inv_d2;
+    o.cb2 = (kSaCb2 / kSaSigma) * rho * grad2;
+    return o;
+}
+
+// Strain-rate magnitude sqrt(2 S_ij S_ij) from the velocity gradient rows
+double strain_rate(const oracle_ctx& c, size_t i) {
+    const size_t n = static_cast<size_t>(c.ntot);
+    const auto G = [&](const std::vector<double>& a, int v) { return a[static_cast<size_t>(v) * n + i]; };
+    const double ux = G(c.gx, 1), uy = G(c.gy, 1), uz = G(c.gz, 1);
+    const double vx = G(c.gx, 2), vy = G(c.gy, 2), vz = G(c.gz, 2);
+    const double wx = G(c.gx, 3), wy = G(c.gy, 3), wz = G(c.gz, 3);
+    const double s_xy = uy + vx, s_xz = uz + wx, s_yz = vz + wy;
+    const double s2 = 2.0 * (ux * ux + vy * vy + wz * wz) + 0.5 * (s_xy * s_xy + s_xz * s_xz + s_yz * s_yz);
+    return std::sqrt(std::max(s2, 0.0));
+}
+
+// apply_bcs (spalart_allmaras.hpp:215-240): ghost rho nu_tilde per zone
+void sa_apply_bcs(oracle_ctx& c) {
+    const double* Q0 = plane(c.q, c.ntot, 0);
+    const double* Q4 = plane(c.q, c.ntot, 4);
+    for (i32 f = c.ninner; f < c.nf; ++f) {
+        const size_t i = static_cast<size_t>(f - c.ninner);
+        const size_t c0 = static_cast<size_t>(c.owner[static_cast<size_t>(f)]);
+        const size_t cg = static_cast<size_t>(c.nc) + i;
+        switch (c.sa.kind[i]) {
+            case ZWall: c.sa.rnu[cg] = -c.sa.rnu[c0]; break;
+            case ZInflow: c.sa.rnu[cg] = c.eos.density_Tp(Q4[cg], Q0[cg]) * c.sa.nu_inf; break;
+            default: c.sa.rnu[cg] = c.sa.rnu[c0]; break;
+        }
+    }
+}
+
+// pre_sweep (spalart_allmaras.hpp:246-270): density, nu_tilde, eddy viscosity
+void sa_pre_sweep(oracle_ctx& c) {
+    const double* Q0 = plane(c.q, c.ntot, 0);
+    const double* Q4 = plane(c.q, c.ntot, 4);
+    for (i32 i = 0; i < c.nc; ++i) {
+        const size_t s = static_cast<size_t>(i);
+        const double rho = c.eos.density_Tp(Q4[s], Q0[s]);
+        const double nu = c.sa.rnu[s] / rho;
+        c.sa.rho[s] = rho;
+        c.sa.nu[s] = nu;
+        const double nu_lam = viscosity(Q4[s]) / rho;
+        c.sa.mut[s] = rho * nu * sa_fv1(nu / nu_lam);
+    }
+    for (i32 g = c.nc; g < c.ntot; ++g) {
+        const size_t s = static_cast<size_t>(g);
+        const double rho_g = c.eos.density_Tp(Q4[s], Q0[s]);
+        c.sa.rho[s] = rho_g;
+        c.sa.nu[s] = c.sa.rnu[s] / rho_g;
+        c.sa.mut[s] = 0.0;
+    }
+}
+
+// compute_gradients (spalart_allmaras.hpp:196-212): the configured gradient
+// method on nu_tilde, then BC ghost planes mirror the owner (sign flip at walls)
+void sa_gradients(oracle_ctx& c) {
+    const size_t n = static_cast<size_t>(c.ntot);
+    const double* s = c.sa.nu.data();
+    double* gx = c.sa.g.data();
+    double* gy = gx + n;
+    double* gz = gy + n;
+    if (c.num.gradient == 2) {  // gradient_operator.cpp:549-599
+        const size_t E = c.ccn.size();
+        const double* wx = c.lsq_w.data();
+        const double* wy = wx + E;
+        const double* wz = wy + E;
+        for (i32 i = 0; i < c.nc; ++i) {
+            const size_t is = static_cast<size_t>(i);
+            double a = 0, b = 0, d = 0;
+            for (i32 j = c.ccn_off[is]; j < c.ccn_off[is + 1]; ++j) {
+                const double dv = s[static_cast<size_t>(c.ccn[static_cast<size_t>(j)])] - s[is];
+                a += wx[j] * dv;
+                b += wy[j] * dv;
+                d += wz[j] * dv;
+            }
+            gx[is] = a;
+            gy[is] = b;
+            gz[is] = d;
+        }
+    } else if (c.num.gradient == 1) {  // gradient_operator.cpp:61-130
+        for (i32 i = 0; i < c.nc; ++i) {
+            const size_t is = static_cast<size_t>(i);
+            double a = 0, b = 0, d = 0;
+            for (i32 e = c.cf_off[is]; e < c.cf_off[is + 1]; ++e) {
+                const size_t f = static_cast<size_t>(c.cf[static_cast<size_t>(e)]);
+                const i32 o = c.owner[f], nb = c.neigh[f];
+                const double s05 = (i == o) ? 0.5 : -0.5;
+                const size_t j = (nb >= 0) ? (i == o ? static_cast<size_t>(nb) : static_cast<size_t>(o))
+                                           : static_cast<size_t>(c.nc) + f - static_cast<size_t>(c.ninner);
+                const double sf = s[is] + s[j];
+                a += s05 * c.area[f] * c.nx[f] * sf;
+                b += s05 * c.area[f] * c.ny[f] * sf;
+                d += s05 * c.area[f] * c.nz[f] * sf;
+            }
+            const double iv = 1.0 / c.vol[is];
+            gx[is] = a * iv;
+            gy[is] = b * iv;
+            gz[is] = d * iv;
+        }
+    } else {  // gradient_operator.cpp:11-59
+        std::fill_n(gx, c.nc, 0.0);
+        std::fill_n(gy, c.nc, 0.0);
+        std::fill_n(gz, c.nc, 0.0);
+        for (i32 f = 0; f < c.nf; ++f) {
+            const size_t fs = static_cast<size_t>(f);
+            const size_t o = static_cast<size_t>(c.owner[fs]);
+            const bool inner = f < c.ninner;
+            const size_t nb = inner ? static_cast<size_t>(c.neigh[fs]) : static_cast<size_t>(c.nc) + fs - static_cast<size_t>(c.ninner);
+            const double a05 = 0.5 * c.area[fs];
+            const double sum = s[o] + s[nb];
+            const double fx = a05 * c.nx[fs] * sum, fy = a05 * c.ny[fs] * sum, fz = a05 * c.nz[fs] * sum;
+            gx[o] += fx;
+            gy[o] += fy;
+            gz[o] += fz;
+            if (inner) {
+                gx[nb] -= fx;
+                gy[nb] -= fy;
+                gz[nb] -= fz;
+            }
+        }
+        for (i32 i = 0; i < c.nc; ++i) {
+            const double iv = 1.0 / c.vol[static_cast<size_t>(i)];
+            gx[i] *= iv;
+            gy[i] *= iv;
+            gz[i] *= iv;
+        }
+    }
+    for (i32 f = c.ninner; f < c.nf; ++f) {
+        const size_t i = static_cast<size_t>(f - c.ninner);
+        const size_t c0 = static_cast<size_t>(c.owner[static_cast<size_t>(f)]);
+        const size_t cg = static_cast<size_t>(c.nc) + i;
+        const double sg = (c.sa.kind[i] == ZWall) ? -1.0 : 1.0;
+        gx[cg] = sg * gx[c0];
+        gy[cg] = sg * gy[c0];
+        gz[cg] = sg * gz[c0];
+    }
+}
+
+// face_gradient (spalart_allmaras.hpp:500-525): averaged + skew-corrected
+void sa_face_gradient(const oracle_ctx& c, size_t f, size_t cl, double nu_l, size_t cr, double nu_r, double gn[3]) {
+    const size_t n = static_cast<size_t>(c.ntot);
+    const double* g = c.sa.g.data();
+    const double dx = c.dvx[f], dy = c.dvy[f], dz = c.dvz[f];
+    gn[0] = 0.5 * (g[cl] + g[cr]);
+    gn[1] = 0.5 * (g[n + cl] + g[n + cr]);
+    gn[2] = 0.5 * (g[2 * n + cl] + g[2 * n + cr]);
+    const double jump = nu_r - nu_l;
+    const double gd = gn[0] * dx + gn[1] * dy + gn[2] * dz;
+    const double inv_d = c.dinv[f];
+    const double corr = (jump - gd) * inv_d * inv_d;
+    gn[0] += corr * dx;
+    gn[1] += corr * dy;
+    gn[2] += corr * dz;
+}
+
+// face_sweep (spalart_allmaras.hpp:279-373): upwind convection on the mean
+// mass flux + diffusion; interior faces add their spectral radius to lam.
+// diag collects the point-implicit diagonal: outflow m / rho and coef A / (d rho)
+// of each face.
+void sa_face_sweep(oracle_ctx& c) {
+    std::fill(c.sa.res.begin(), c.sa.res.end(), 0.0);
+    std::fill(c.sa.diag.begin(), c.sa.diag.end(), 0.0);
+    const double inv_sigma = 1.0 / kSaSigma;
+    const double* Q4 = plane(c.q, c.ntot, 4);
+    const std::vector<double>& nu = c.sa.nu;
+    const std::vector<double>& rho = c.sa.rho;
+    for (i32 f = 0; f < c.ninner; ++f) {
+        const size_t fs = static_cast<size_t>(f);
+        const size_t c0 = static_cast<size_t>(c.owner[fs]), c1 = static_cast<size_t>(c.neigh[fs]);
+        const double m = c.sa.mdot[fs];
+        const double nu_up = (m >= 0.0) ? nu[c0] : nu[c1];
+        const double f_conv = m * nu_up;
+        double gn[3];
+        sa_face_gradient(c, fs, c0, nu[c0], c1, nu[c1], gn);
+        const double Tf = 0.5 * (Q4[c0] + Q4[c1]);
+        const double mu_f = viscosity(Tf);
+        const double rho_f = 0.5 * (rho[c0] + rho[c1]);
+        const double nu_f = 0.5 * (nu[c0] + nu[c1]);
+        const double coef = inv_sigma * (mu_f + rho_f * nu_f);
+        const double f_diff = coef * (gn[0] * c.nx[fs] + gn[1] * c.ny[fs] + gn[2] * c.nz[fs]) * c.area[fs];
+        const double flux = f_conv - f_diff;  // correction 1
+        c.sa.res[c0] += flux;
+        c.sa.res[c1] -= flux;
+        const double visc_term = 2.0 * inv_sigma * (mu_f + rho_f * nu_f) / rho_f * c.area[fs] * c.dinv[fs];
+        const double w = std::fabs(m) / rho_f + visc_term;
+        c.lam[c0] += w;
+        c.lam[c1] += w;
+        const double dcoef = coef * c.area[fs] * c.dinv[fs];
+        c.sa.diag[c0] += (std::max(m, 0.0) + dcoef) / rho[c0];
+        c.sa.diag[c1] += (std::max(-m, 0.0) + dcoef) / rho[c1];
+    }
+    for (i32 f = c.ninner; f < c.nf; ++f) {
+        const size_t fs = static_cast<size_t>(f);
+        const size_t i = fs - static_cast<size_t>(c.ninner);
+        const size_t c0 = static_cast<size_t>(c.owner[fs]);
+        const size_t cg = static_cast<size_t>(c.nc) + i;
+        const double m = c.sa.mdot[fs];
+        const int kind = c.sa.kind[i];
+        const bool inflow = m < 0.0 && (kind == ZInflow || kind == ZOutflow);  // correction 3
+        const double nu_up = inflow ? c.sa.nu_inf : nu[c0];
+        const double f_conv = m * nu_up;
+        const double rho_g = rho[cg];
+        const double nu_g = inflow ? c.sa.nu_inf : c.sa.rnu[cg] / rho_g;
+        double gn[3];
+        sa_face_gradient(c, fs, c0, nu[c0], cg, nu_g, gn);
+        const double Tf = 0.5 * (Q4[c0] + Q4[cg]);
+        const double mu_f = viscosity(Tf);
+        const double rho_f = 0.5 * (rho[c0] + rho_g);
+        const double coef = inv_sigma * (mu_f + rho_f * 0.5 * (nu[c0] + nu_g));
+        const double f_diff = coef * (gn[0] * c.nx[fs] + gn[1] * c.ny[fs] + gn[2] * c.nz[fs]) * c.area[fs];
+        c.sa.res[c0] += f_conv - f_diff;  // correction 1
+        c.sa.diag[c0] += (std::max(m, 0.0) + coef * c.area[fs] * c.dinv[fs]) / rho[c0];
+    }
+}
+
+// cell_sources (spalart_allmaras.hpp:382-432); correction 2 on the cb2 sign.
+// diag gains V max(2 cw1 fw nu / d^2 - cb1 Sbar, 0).
+void sa_cell_sources(oracle_ctx& c) {
+    const double* Q4 = plane(c.q, c.ntot, 4);
+    const size_t n = static_cast<size_t>(c.ntot);
+    for (i32 i = 0; i < c.nc; ++i) {
+        const size_t s = static_cast<size_t>(i);
+        const double* g = c.sa.g.data();
+        const double grad2 = g[s] * g[s] + g[n + s] * g[n + s] + g[2 * n + s] * g[2 * n + s];
+        const SaSource o = sa_source(c.sa.rho[s], c.sa.nu[s], viscosity(Q4[s]), c.sa.dist[s], strain_rate(c, s), grad2);
+        c.sa.res[s] += c.vol[s] * (o.D - o.P - o.cb2);
+        const double d = std::max(c.sa.dist[s], kSaMinWallDist);
+        const double lin = 2.0 * kSaCw1 * o.fw * c.sa.nu[s] / (d * d) - kSaCb1 * o.sbar;
+        c.sa.diag[s] += c.vol[s] * std::max(lin, 0.0);
+    }
+}
+
+// Module update after the mean-flow step.  Frozen: the reference's implicit
+// Euler copies the module block unchanged (implicit_euler.hpp:64-72); point
+// implicit: rnu -= res / (V / dt + diag).  Both end with the positivity clamp
+// of post_stage (spalart_allmaras.hpp:435-442).
+void sa_update(oracle_ctx& c) {
+    for (i32 i = 0; i < c.nc; ++i) {
+        const size_t s = static_cast<size_t>(i);
+        double v = c.sa.rnu[s];
+        if (c.sa.cfg.update == 1) v = v - c.sa.res[s] / (c.alpha[s] + c.sa.diag[s]);
+        c.sa.rnu[s] = (v < 0.0) ? 0.0 : v;
+    }
+}
+
 // solver.hpp:169-247
 void evaluate_residual(oracle_ctx& c) {
     for (i32 i = 0; i < c.nc; ++i) {
@@ -1075,6 +1402,10 @@ void evaluate_residual(oracle_ctx& c) {
                      plane(c.q, c.ntot, 3)[is], plane(c.q, c.ntot, 4)[is]);
     }
     update_ghosts(c);
+    if (c.sa_on()) {
+        sa_apply_bcs(c);
+        sa_pre_sweep(c);
+    }
     const bool need_grad = c.num.reconstruction == 1 || c.num.flow_model == 1;
     if (need_grad) {
         if (c.num.gradient == 0) gradient_ggfb(c);
@@ -1082,8 +1413,13 @@ void evaluate_residual(oracle_ctx& c) {
         else gradient_lsqn(c);
         update_ghost_grads(c);
         if (c.num.reconstruction == 1) compute_limiters(c);
+        if (c.sa_on()) sa_gradients(c);
     }
     compute_fluxes(c);
+    if (c.sa_on()) {
+        sa_face_sweep(c);
+        sa_cell_sources(c);
+    }
     c.have_residual = true;
 }
 
@@ -1118,7 +1454,10 @@ void assemble_jacobian(oracle_ctx& c) {
         hllc_jacobian(c.eos, UL, UR, c.nx[fs], c.ny[fs], c.nz[fs], c.area[fs], dFL, dFR);
         if (visc) {
             double vL[25], vR[25];
-            viscous_jacobian(c.eos, c.dinv[fs], qL, qR, 0.0, 0.0, c.area[fs], vL, vR, c.num.prandtl, c.num.prandtl_turb);
+            // jacobian_kernel.hpp:156-157, 209-210, 252-254: ghost side zero
+            const double mutL = c.sa_on() ? c.sa.mut[c0] : 0.0;
+            const double mutR = (c.sa_on() && inner) ? c.sa.mut[c1] : 0.0;
+            viscous_jacobian(c.eos, c.dinv[fs], qL, qR, mutL, mutR, c.area[fs], vL, vR, c.num.prandtl, c.num.prandtl_turb);
             for (int i = 0; i < 25; ++i) {
                 dFL[i] += vL[i];
                 if (inner) dFR[i] += vR[i];
@@ -1645,6 +1984,7 @@ int oracle_implicit_step(oracle_ctx* c, const oracle_linparams* lp, oracle_step_
         oracle_residual_norms(c, l2);
         for (i32 i = 0; i < c->nc; ++i)
             for (int v = 0; v < 5; ++v) plane(c->U, c->ntot, v)[i] = plane(c->U, c->ntot, v)[i] + c->du[static_cast<size_t>(i) * 5 + static_cast<size_t>(v)];
+        if (c->sa_on()) sa_update(*c);
         if (out) {
             for (int v = 0; v < 5; ++v) out->l2[v] = l2[v];
             out->linear = lr;
@@ -1743,6 +2083,94 @@ int oracle_boundary_integrals(oracle_ctx* c, double* mass, double* energy, doubl
 // <= rel_tol or max_sweeps sweeps have run.
 int oracle_wall_distance(oracle_ctx* c, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist) {
     if (!c || !dist) return -1;
+    wall_distance(*c, stencil, max_sweeps, rel_tol, dist);
+    return 0;
+}
+
+int oracle_set_turbulence(oracle_ctx* c, const oracle_turbulence* t) {
+    if (!c || !t) return -1;
+    if (t->model == 0) {
+        c->sa.cfg = *t;
+        return 0;
+    }
+    if (t->model != 1 || (t->update != 0 && t->update != 1) || c->num.flow_model != 1 || !(t->nu_inf_ratio > 0.0) ||
+        !(t->rho_inf > 0.0) || !(t->p_inf > 0.0) || t->max_distance_sweeps < 0) {
+        c->err = "invalid turbulence parameters (SA needs the viscous flow model)";
+        return -1;
+    }
+    auto& s = c->sa;
+    s.cfg = *t;
+    const size_t nt = static_cast<size_t>(c->ntot), nc = static_cast<size_t>(c->nc);
+    s.kind.assign(static_cast<size_t>(c->nbf), ZSymmetry);
+    for (i32 f = c->ninner; f < c->nf; ++f) {
+        const i32 p = c->patch[static_cast<size_t>(f)];
+        const int ty = (p >= 0 && p < c->npatch) ? c->patch_type[static_cast<size_t>(p)] : Sym;
+        s.kind[static_cast<size_t>(f - c->ninner)] = nu_zone(ty);
+    }
+    // initialize (spalart_allmaras.hpp:157-193): wall distance over the vertex
+    // stencil, freestream working values
+    s.dist.assign(nc, 0.0);
+    wall_distance(*c, 1, t->max_distance_sweeps, t->distance_tolerance, s.dist.data());
+    const double T_inf = c->eos.temperature_rhop(t->rho_inf, t->p_inf);
+    s.nu_inf = t->nu_inf_ratio * viscosity(T_inf) / t->rho_inf;
+    s.rho_nu_inf = t->rho_inf * s.nu_inf;
+    s.rnu.assign(nt, s.rho_nu_inf);  // init_state (spalart_allmaras.hpp:184-192)
+    s.nu.assign(nt, 0.0);
+    s.rho.assign(nt, 0.0);
+    s.mut.assign(nt, 0.0);
+    s.g.assign(3 * nt, 0.0);
+    s.res.assign(nt, 0.0);
+    s.diag.assign(nt, 0.0);
+    s.mdot.assign(static_cast<size_t>(c->nf), 0.0);
+    return 0;
+}
+
+int oracle_set_turbulence_state(oracle_ctx* c, const double* rnu) {
+    if (!c || !rnu || !c->sa_on()) return -1;
+    std::copy(rnu, rnu + c->nc, c->sa.rnu.begin());
+    return 0;
+}
+
+int oracle_get_turbulence(oracle_ctx* c, double* rnu, double* nu, double* mut, double* dist, double* res, double* grad,
+                          double* mdot, double* diag) {
+    if (!c || !c->sa_on()) return -1;
+    const size_t nc = static_cast<size_t>(c->nc), nt = static_cast<size_t>(c->ntot);
+    const auto out = [&](const std::vector<double>& a, double* o) {
+        if (o) std::copy(a.begin(), a.begin() + static_cast<long>(nc), o);
+    };
+    out(c->sa.rnu, rnu);
+    out(c->sa.nu, nu);
+    out(c->sa.mut, mut);
+    out(c->sa.dist, dist);
+    out(c->sa.res, res);
+    out(c->sa.diag, diag);
+    if (grad)
+        for (size_t d = 0; d < 3; ++d) std::copy(c->sa.g.begin() + static_cast<long>(d * nt), c->sa.g.begin() + static_cast<long>(d * nt + nc), grad + d * nc);
+    if (mdot) std::copy(c->sa.mdot.begin(), c->sa.mdot.end(), mdot);
+    return 0;
+}
+
+int oracle_sa_point(double rho, double nu, double mu_lam, double d, double S, double grad2, double out[8]) {
+    if (!out) return -1;
+    const SaSource o = sa_source(rho, nu, mu_lam, d, S, grad2);
+    const double v[8] = {o.P, o.D, o.cb2, o.fv1, o.fv2, o.sbar, o.r, o.fw};
+    std::copy(v, v + 8, out);
+    return 0;
+}
+
+double oracle_sa_constant(int32_t which) {
+    const double v[8] = {kSaCb1, kSaCb2, kSaSigma, kSaKappa, kSaCw1, kSaCw2, kSaCw3, kSaCv1};
+    return (which >= 0 && which < 8) ? v[which] : 0.0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// aux_geometry.cpp:545-697 (single rank): seeds at the no-slip wall faces, then
+// forward + backward relaxation sweeps over the face (0) or vertex (1) stencil
+void wall_distance(oracle_ctx& cr, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist) {
+    oracle_ctx* c = &cr;
     const double kInf = 1.0e30, kMinDist = 1.0e-12;
     std::vector<double> d(static_cast<size_t>(c->nc), kInf);
     for (i32 f = c->ninner; f < c->nf; ++f) {
@@ -1785,8 +2213,11 @@ int oracle_wall_distance(oracle_ctx* c, int32_t stencil, int32_t max_sweeps, dou
         if (max_rel <= rel_tol) break;
     }
     std::copy(d.begin(), d.end(), dist);
-    return 0;
 }
+
+}  // namespace
+
+extern "C" {
 
 int oracle_spmv(oracle_ctx* c, const double* x, double* y) {
     if (!c || !x || !y) return -1;

@@ -64,6 +64,15 @@ typedef struct {
     oracle_iter_result linear;
 } oracle_step_result;
 
+/* Turbulence module (layout of cfdg_turbulence) */
+typedef struct {
+    int32_t model, update;
+    double nu_inf_ratio;
+    int32_t max_distance_sweeps, reserved;
+    double distance_tolerance;
+    double rho_inf, p_inf;
+} oracle_turbulence;
+
 typedef struct oracle_ctx oracle_ctx;
 
 int oracle_create(const oracle_mesh_view* mesh, const oracle_numerics* num, const oracle_bc* bcs,
@@ -110,6 +119,19 @@ int oracle_sgs_apply(oracle_ctx* ctx, int32_t sgs_order, const double* r, double
 int oracle_spmv(oracle_ctx* ctx, const double* x, double* y);
 int oracle_boundary_integrals(oracle_ctx* ctx, double* mass, double* energy, double* force, int32_t n_patches);
 int oracle_wall_distance(oracle_ctx* ctx, int32_t stencil, int32_t max_sweeps, double rel_tol, double* dist);
+
+/* Spalart-Allmaras module: set up (zone kinds, wall distance over the vertex
+   stencil, freestream nu_tilde = ratio mu(T_inf) / rho_inf, state = rho_inf
+   nu_inf), state, and the per-evaluation fields of the last residual (n_cells
+   each; grad 3 x n_cells; mdot n_faces; diag the point-implicit diagonal). */
+int oracle_set_turbulence(oracle_ctx* ctx, const oracle_turbulence* t);
+int oracle_set_turbulence_state(oracle_ctx* ctx, const double* rho_nu);
+int oracle_get_turbulence(oracle_ctx* ctx, double* rho_nu, double* nu, double* mut, double* dist, double* res,
+                          double* grad, double* mdot, double* diag);
+/* One cell's source terms: out = P, D, cb2 term, fv1, fv2, Sbar, r, fw */
+int oracle_sa_point(double rho, double nu, double mu_lam, double d, double S, double grad2, double out[8]);
+/* Model constants: cb1, cb2, sigma, kappa, cw1, cw2, cw3, cv1 */
+double oracle_sa_constant(int32_t which);
 
 #ifdef __cplusplus
 }

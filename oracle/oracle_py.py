@@ -38,6 +38,9 @@ def _load():
         "oracle_row_groups": [vp, vp, vp, vp, vp, vp, vp], "oracle_sgs_apply": [vp, i32, vp, vp],
         "oracle_spmv": [vp, vp, vp], "oracle_boundary_integrals": [vp, vp, vp, vp, i32],
         "oracle_wall_distance": [vp, i32, i32, d, vp],
+        "oracle_set_turbulence": [vp, vp], "oracle_set_turbulence_state": [vp, vp],
+        "oracle_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "oracle_sa_point": [d, d, d, d, d, d, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -45,6 +48,8 @@ def _load():
         fn.restype = C.c_int
     lib.oracle_last_error.argtypes = [vp]
     lib.oracle_last_error.restype = C.c_char_p
+    lib.oracle_sa_constant.argtypes = [i32]
+    lib.oracle_sa_constant.restype = d
     return lib
 
 
@@ -203,3 +208,37 @@ class Oracle:
         mass, energy, force = np.empty(n), np.empty(n), np.empty((n, 3))
         self._chk(lib.oracle_boundary_integrals(self._h, _ptr(mass), _ptr(energy), _ptr(force), n))
         return mass, energy, force
+
+    def set_turbulence(self, turb):
+        """turb: cfdg.Turbulence (identical layout)."""
+        self._chk(lib.oracle_set_turbulence(self._h, C.byref(turb)))
+
+    def set_turbulence_state(self, rho_nu):
+        rho_nu = np.ascontiguousarray(rho_nu, dtype=np.float64)
+        self._chk(lib.oracle_set_turbulence_state(self._h, _ptr(rho_nu)))
+
+    def turbulence(self):
+        """Fields of the last residual evaluation: dict of rho_nu, nu, mut,
+        dist, res, diag (n_cells), grad (3, n_cells), mdot (n_faces)."""
+        n = self.n_cells
+        out = {k: np.empty(n) for k in ("rho_nu", "nu", "mut", "dist", "res", "diag")}
+        out["grad"] = np.empty((3, n))
+        out["mdot"] = np.empty(self.mesh.n_faces)
+        self._chk(lib.oracle_get_turbulence(self._h, *(_ptr(out[k]) for k in
+                                                       ("rho_nu", "nu", "mut", "dist", "res", "grad", "mdot", "diag"))))
+        return out
+
+
+SA_CONSTANTS = ("cb1", "cb2", "sigma", "kappa", "cw1", "cw2", "cw3", "cv1")
+
+
+def sa_constants():
+    return {k: lib.oracle_sa_constant(i) for i, k in enumerate(SA_CONSTANTS)}
+
+
+def sa_point(rho, nu, mu_lam, d, S, grad2):
+    """One cell's SA source evaluation: dict of P, D, cb2, fv1, fv2, sbar, r, fw."""
+    out = np.empty(8)
+    if lib.oracle_sa_point(rho, nu, mu_lam, d, S, grad2, _ptr(out)) != 0:
+        raise OracleError("oracle_sa_point failed")
+    return dict(zip(("P", "D", "cb2", "fv1", "fv2", "sbar", "r", "fw"), out))

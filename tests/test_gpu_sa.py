@@ -1,0 +1,139 @@
+"""Spalart-Allmaras module on the GPU against the CPU oracle.
+
+Case: flat plate 24x16x8 at Re_L = 5e6 (viscous + SA, node LSQ, MUSCL),
+freestream with 2 % seeded noise in U and 20 % in rho nu_tilde; a second run
+uses Green-Gauss gradients so every gradient path of the module is covered.
+
+Tolerances (FP64 on both sides, same summation order):
+  * eddy viscosity, nu_tilde, its gradient, density: 1e-12 relative;
+  * face mass flux: 1e-12 of max |mdot|;
+  * module residual and point-implicit diagonal: 1e-12 of max |res|, max |diag|;
+  * mean-flow residual with mu_t: 1e-12 of the per-cell flux scale; spectral
+    radius with the module's contribution, dt and alpha: 1e-12 relative;
+  * Jacobian blocks with mu_t: 1e-12 of max |B|;
+  * wall distance: bit-identical (same relaxation, same order);
+  * five point-implicit steps: state and rho nu_tilde within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import cfdg
+import cfdg.cases as cases
+from helpers import assert_close_scaled, perturbed_state, rel_err
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(gradient="LEAST_SQUARES_NODE"):
+    return cases.plate(ni=24, nj=16, nk=8, re=5.0e6, gradient=gradient)
+
+
+def _pair(case, update=cfdg.TURB_POINT_IMPLICIT, seed=3):
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    o = Oracle(mesh, case.numerics, case.bcs)
+    U = perturbed_state(s, case, seed=seed)
+    turb = cfdg.Turbulence.make(rho_inf=case.rho, p_inf=case.p, update=update)
+    s.set_turbulence(turb)
+    o.set_turbulence(turb)
+    rnu0 = s.turbulence()["rho_nu"]
+    rng = np.random.default_rng(seed + 100)
+    rnu = rnu0 * (1.0 + 0.2 * rng.standard_normal(rnu0.shape))
+    for x in (s, o):
+        x.set_state(U)
+        x.set_turbulence_state(rnu)
+    return mesh, s, o
+
+
+@pytest.mark.parametrize("gradient", ["LEAST_SQUARES_NODE", "GREEN_GAUSS_FACE", "GREEN_GAUSS_CELL"])
+def test_sa_residual_matches_oracle(gradient):
+    case = _case(gradient)
+    mesh, s, o = _pair(case)
+    s.eval_residual()
+    o.eval_residual()
+    tg, to = s.turbulence(), o.turbulence()
+    assert np.array_equal(tg["dist"], to["dist"])
+    for k in ("nu", "mut"):
+        assert rel_err(tg[k], to[k]).max() <= 1e-12, k
+    assert_close_scaled(tg["grad"], to["grad"], np.abs(to["grad"]).max(), 1e-12, "nu_tilde gradient")
+    assert_close_scaled(tg["mdot"], to["mdot"], np.abs(to["mdot"]).max(), 1e-12, "mass flux")
+    assert_close_scaled(tg["res"], to["res"], np.abs(to["res"]).max(), 1e-12, "module residual")
+    assert_close_scaled(tg["diag"], to["diag"], np.abs(to["diag"]).max(), 1e-12, "module diagonal")
+    Rg, lg = s.residual()
+    Ro, lo = o.residual()
+    assert_close_scaled(Rg, Ro, o.flux_scale(), 1e-12, "mean residual with mu_t")
+    assert rel_err(lg, lo).max() <= 1e-12
+
+
+def test_sa_jacobian_and_dt_match_oracle():
+    mesh, s, o = _pair(_case())
+    for x in (s, o):
+        x.eval_residual()
+        x.compute_dt()
+        x.assemble_jacobian()
+    (dtg, ag), (dto, ao) = s.dt(), o.dt()
+    assert rel_err(dtg, dto).max() <= 1e-12 and rel_err(ag, ao).max() <= 1e-12
+    Bg, Bo = s.bsr_values(), o.bsr_values()
+    assert_close_scaled(Bg, Bo, np.abs(Bo).max(), 1e-12, "Jacobian blocks with mu_t")
+
+
+@pytest.mark.parametrize("update", [cfdg.TURB_POINT_IMPLICIT, cfdg.TURB_FROZEN])
+def test_sa_implicit_steps_match_oracle(update):
+    mesh, s, o = _pair(_case(), update=update)
+    lp = cases.default_linparams()
+    for _ in range(5):
+        s.implicit_step(lp)
+        o.implicit_step(lp)
+    assert rel_err(s.get_state(), o.get_state()).max() <= 1e-10
+    rg, ro = s.turbulence()["rho_nu"], o.turbulence()["rho_nu"]
+    assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
+    assert np.all(rg >= 0.0)
+
+
+def test_sa_steady_state_matches_oracle():
+    """Plate 16x8x4, Re_L 5e6, unlimited MUSCL, CFL 1000, point-implicit SA,
+    from 2 % noise: both sides run to a relative residual of 1e-11; primitive
+    fields within 1e-8 of their freestream scales, rho nu_tilde within 1e-8 of
+    its freestream value, wall force within 1e-8 relative."""
+    case = cases.plate(ni=16, nj=8, nk=4, re=5.0e6, limiter="UNLIMITED", cfl=1000.0)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    o = Oracle(mesh, case.numerics, case.bcs)
+    U0 = perturbed_state(o, case)
+    turb = cfdg.Turbulence.make(rho_inf=case.rho, p_inf=case.p)
+    o.set_turbulence(turb)
+    o.set_state(U0)
+    lp = cases.default_linparams()
+    l0 = None
+    for _ in range(8000):
+        l2 = np.array(o.implicit_step(lp).l2)
+        l0 = l2 if l0 is None else l0
+        if np.max(l2 / l0) <= 1e-11:
+            break
+    else:
+        pytest.fail("oracle did not converge")
+    o.eval_residual()
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s.init_uniform(case.rho, case.velocity, case.p)
+    s.set_turbulence(turb)
+    s.set_state(U0)
+    r = s.run_steady(lp, 8000, rel_tol=1e-11)
+    assert r.status == cfdg.CONVERGED, (r.status, r.steps, r.rel)
+    s.eval_residual()
+
+    def prim(U):
+        rho = U[0]
+        u, v, w = U[1] / rho, U[2] / rho, U[3] / rho
+        return np.stack([rho, u, v, w, (cases.GAMMA - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w))])
+
+    speed = float(np.linalg.norm(case.velocity))
+    scale = np.array([case.rho, speed, speed, speed, case.p])[:, None]
+    err = np.max(np.abs(prim(s.get_state()) - prim(o.get_state())) / scale, axis=1)
+    assert np.all(err <= 1e-8), err
+    tg, to = s.turbulence(), o.turbulence()
+    rnu_inf = to["rho_nu"].max()
+    assert np.abs(tg["rho_nu"] - to["rho_nu"]).max() <= 1e-8 * rnu_inf
+    walls = [p for p, b in enumerate(case.bcs) if b.type in (cfdg.BC["NO_SLIP_WALL"], cfdg.BC["NO_SLIP_WALL_HEAT_FLUX"])]
+    fg = s.boundary_integrals()[2][walls].sum(axis=0)
+    fo = o.boundary_integrals()[2][walls].sum(axis=0)
+    assert np.linalg.norm(fg - fo) <= 1e-8 * np.linalg.norm(fo), (fg, fo)
