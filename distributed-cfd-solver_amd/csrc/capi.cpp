@@ -457,8 +457,12 @@ cfdg_iter_result defect_correction(cfdg_ctx& c, const cfdg_linparams& lp, const 
         cfdg::launch_cast_f64_to_f32(N, c.r.get(), c.f32.b.get(), c.stream);
         c.f32.x.zero(c.stream);
         cfdg_linparams in = lp;
+        // stop the inner solve at max(inner_rel_tol ||r||, 0.9 tol): no further
+        // than the caller's tolerance needs (the FP64 outer residual of the
+        // update equals the inner residual up to FP32 rounding), and at least a
+        // factor inner_rel_tol per pass when the tolerance lies beyond FP32 reach
         in.rel_tol = c.prec.inner_rel_tol;
-        in.abs_tol = 0.5 * tol;
+        in.abs_tol = 0.9 * tol;
         in.max_iter = lp.max_iter - total;
         const cfdg_iter_result ir = bicgstab<float>(c, in, P, w, c.f32.x.get(), c.f32.b.get());
         total += ir.iterations;

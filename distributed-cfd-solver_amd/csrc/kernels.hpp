@@ -18,11 +18,7 @@ namespace cfdg {
 
 constexpr int kMaxPatches = 64;
 constexpr int kTileCells = 256;       // cells per tile (= threads per flux workgroup)
-constexpr int kTileFaces = 1024;      // most faces touching one tile
-#ifndef CFDG_FLUX_CHUNK
-#define CFDG_FLUX_CHUNK 512
-#endif
-constexpr int kFluxChunk = CFDG_FLUX_CHUNK;  // faces per LDS chunk of the flux kernel (6 doubles each)
+constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux kernel
 constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kGradSlots = 640;       // LDS slots (tile cells + stencil halo) of the staged gradient kernel
 // storage of the Jacobian blocks (values of the precision enumeration of cfdg.h)

@@ -450,62 +450,42 @@ __device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm,
 // parallel) into LDS, then each cell adds its faces' fluxes in ascending face
 // index, with + for the owner and - for the neighbour (residual_kernel.hpp:
 // 175-306 formed as a gather; only faces on a tile boundary are evaluated by
-// both adjacent tiles).  The tile's face list (ascending face index) is taken
-// in chunks of kFluxChunk faces, so the LDS holds one chunk: a cell's faces
-// occupy ascending list positions, so it consumes them chunk by chunk in the
-// same ascending order, and the smaller LDS footprint lets more workgroups
-// share a CU.  With turbulence the kernel also records each face's mass flux
-// F[0] (from the tile that owns the face's owner cell).
+// both adjacent tiles).  With turbulence the kernel also records each face's
+// mass flux F[0] (from the tile that owns the face's owner cell).
 template <bool kTurb>
 __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
-    __shared__ double sF[6][kFluxChunk];
-    __shared__ double sAcc[6][kTileCells];  // per-cell running sums between chunks
-    __shared__ int sPos[kTileCells];        // next entry of each cell's face list
+    __shared__ double sF[6][kTileFaces];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
-    const int c = m.tile_off[t] + threadIdx.x;
-    const bool own = c < m.tile_off[t + 1];
-    for (int v = 0; v < 6; ++v) sAcc[v][threadIdx.x] = 0.0;
-    sPos[threadIdx.x] = own ? m.cf_off[c] : 0;
-    for (int base = 0; base < nfl; base += kFluxChunk) {
-        const int n = (nfl - base < kFluxChunk) ? nfl - base : kFluxChunk;
-        for (int i = threadIdx.x; i < n; i += kTileCells) {
-            double F[5], lam_f;
-            const int j = fb + base + i;
-            const int fc = m.tile_faces[j];
-            const int2 cc = m.tcells[j];
-            face_flux<kTurb>(f, nm, FaceRef{m.tf, m.tld, j}, fc < m.ninner, cc.x, cc.y, F, lam_f);
-            if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
-            for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
-            sF[5][i] = lam_f;
-            if (fc >= m.ninner)  // a boundary face lies in exactly one tile: single writer
-                for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(fc - m.ninner) * 5 + v] = F[v];
-        }
-        __syncthreads();
-        if (own) {
-            double acc[6];
-            for (int v = 0; v < 6; ++v) acc[v] = sAcc[v][threadIdx.x];
-            int e = sPos[threadIdx.x];
-            const int ee = m.cf_off[c + 1];
-            for (; e < ee; ++e) {
-                const int s = m.cf_slot[e];
-                const int k = ((s >= 0) ? s : -s - 1) - base;
-                if (k >= n) break;  // the cell's next face lies in a later chunk
-                if (s >= 0) {
-                    for (int v = 0; v < 5; ++v) acc[v] += sF[v][k];
-                } else {
-                    for (int v = 0; v < 5; ++v) acc[v] -= sF[v][k];
-                }
-                acc[5] += sF[5][k];
-            }
-            for (int v = 0; v < 6; ++v) sAcc[v][threadIdx.x] = acc[v];
-            sPos[threadIdx.x] = e;
-        }
-        __syncthreads();
+    for (int i = threadIdx.x; i < nfl; i += kTileCells) {
+        double F[5], lam_f;
+        const int fc = m.tile_faces[fb + i];
+        const int2 cc = m.tcells[fb + i];
+        face_flux<kTurb>(f, nm, FaceRef{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
+        if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
+        for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
+        sF[5][i] = lam_f;
+        if (fc >= m.ninner)  // a boundary face lies in exactly one tile: single writer
+            for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(fc - m.ninner) * 5 + v] = F[v];
     }
-    if (!own) return;
-    for (int v = 0; v < 5; ++v) f.res[static_cast<long long>(v) * m.nc + c] = sAcc[v][threadIdx.x];
-    f.lam[c] = sAcc[5][threadIdx.x];
+    __syncthreads();
+    const int c = m.tile_off[t] + threadIdx.x;
+    if (c >= m.tile_off[t + 1]) return;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double lam = 0.0;
+    for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
+        const int s = m.cf_slot[e];
+        if (s >= 0) {
+            for (int v = 0; v < 5; ++v) acc[v] += sF[v][s];
+            lam += sF[5][s];
+        } else {
+            const int k = -s - 1;
+            for (int v = 0; v < 5; ++v) acc[v] -= sF[v][k];
+            lam += sF[5][k];
+        }
+    }
+    for (int v = 0; v < 5; ++v) f.res[static_cast<long long>(v) * m.nc + c] = acc[v];
+    f.lam[c] = lam;
 }
 
 // Implicit local time step and diagonal shift (solver.hpp:256-287)

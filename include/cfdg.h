@@ -285,7 +285,8 @@ int cfdg_run_steady(cfdg_ctx* ctx, const cfdg_linparams* lp, const cfdg_steady_p
    diagonal LU is factorised in FP64 from the stored blocks and kept in FP32),
    and cfdg_linear_solve becomes an FP64 defect correction: r = b - A x in FP64
    arithmetic over the stored blocks, an FP32 BiCGSTAB + FP32 SGS solve of
-   A e = r to inner_rel_tol, x += e, repeated until ||r|| meets the caller's
+   A e = r to max(inner_rel_tol ||r||, 0.9 tol) with tol the caller's
+   tolerance below, x += e, repeated until ||r|| meets the caller's
    max(abs_tol, rel_tol ||b||) or max_iter inner iterations / max_outer passes
    are spent.  iterations counts inner iterations; final_residual is the FP64
    ||b - A x||.  Dot products accumulate in FP64 in every form.  The flux
@@ -299,7 +300,7 @@ typedef struct {
     int32_t krylov;         /* CFDG_PREC_FP64 (needs FP64 blocks) | CFDG_PREC_FP32 */
     int32_t max_outer;      /* defect-correction passes (>= 1) */
     int32_t reserved;       /* 0 */
-    double inner_rel_tol;   /* each inner solve stops at inner_rel_tol ||r|| (0 < tol < 1) */
+    double inner_rel_tol;   /* each inner solve stops at max(inner_rel_tol ||r||, 0.9 tol) (0 < inner_rel_tol < 1) */
 } cfdg_precision;
 
 /* Selects the form; reallocates the block storage (the next solve needs a new

@@ -12,7 +12,9 @@ Tolerances (FP64 on both sides, same summation order):
     radius with the module's contribution, dt and alpha: 1e-12 relative;
   * Jacobian blocks with mu_t: 1e-12 of max |B|;
   * wall distance: bit-identical (same relaxation, same order);
-  * five point-implicit steps: state and rho nu_tilde within 1e-10 relative.
+  * five implicit steps (frozen and point-implicit module): state and rho nu_tilde
+    within 1e-9 of each variable's maximum (the BiCGSTAB dot products are tree
+    reductions, 1e-9 per solve as in the mean-flow step test).
 """
 import numpy as np
 import pytest
@@ -85,9 +87,10 @@ def test_sa_implicit_steps_match_oracle(update):
     for _ in range(5):
         s.implicit_step(lp)
         o.implicit_step(lp)
-    assert rel_err(s.get_state(), o.get_state()).max() <= 1e-10
+    Ug, Uo = s.get_state(), o.get_state()
+    assert_close_scaled(Ug, Uo, np.abs(Uo).max(axis=1, keepdims=True), 1e-9, "state after 5 steps")
     rg, ro = s.turbulence()["rho_nu"], o.turbulence()["rho_nu"]
-    assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
+    assert np.abs(rg - ro).max() <= 1e-9 * np.abs(ro).max()
     assert np.all(rg >= 0.0)
 
 
