@@ -7,6 +7,9 @@ plate -- laminar flat plate, 256x128x64 hexahedra, wall-normal growth 1.08,
          Re_L = 1e5, adiabatic no-slip plate for x > 0.
 bump  -- tetrahedral (six Kuhn tetrahedra per hexahedron) channel with a
          Gaussian bottom bump, +-5 % jitter, Mach 0.8, Re = 1e6.
+ramp_sa, plate_sa, bump_sa -- the same geometries with viscous flow and the
+         Spalart-Allmaras module (ramp: no-slip bottom, Re_L = 5e6; plate at
+         Re_L = 5e6); turbulence(case) gives the module's parameter block.
 
 Every case accepts a size override so the same physics runs at parity-test
 sizes (a few thousand cells) and at benchmark sizes.
@@ -158,4 +161,54 @@ def bump(ni=150, nj=150, nk=150, mach=0.8, re=1.0e6, T=300.0, seed=7, jitter=0.0
                 info=dict(mach=mach, re=re, T=T, mu=mu))
 
 
-CASES = {"ramp": ramp, "plate": plate, "bump": bump}
+def ramp_sa(ni=48, nj=16, nk=4, mach=2.5, re=5.0e6, T=300.0, p=101325.0, **num_kw):
+    """The ramp geometry with viscous flow and the Spalart-Allmaras module at
+    Re_L = 5e6 (L = 3 m, the ramp length): no-slip ramp and bottom wall,
+    supersonic inflow / outflow, slip top wall, symmetry z-faces.  The
+    density follows from Re_L; the pressure from the ideal gas at T."""
+    a = sound_speed(T)
+    u = mach * a
+    L = 3.0
+    rho = re * sutherland(T) / (u * L)
+    p = rho * R_AIR * T
+    mesh = dict(kind=0, ni=ni, nj=nj, nk=nk, x0=0.0, x1=3.0, y0=0.0, y1=1.0, z0=0.0, z1=0.25,
+                ramp_x=0.5, ramp_deg=15.0)
+    bcs = [
+        make_bc("SUPERSONIC_INLET", p=p, t=T, velocity=(u, 0.0, 0.0)),
+        make_bc("SUPERSONIC_OUTLET"),
+        make_bc("NO_SLIP_WALL_ADIABATIC"),
+        make_bc("SLIP_WALL"),
+        make_bc("SYMMETRY"),
+        make_bc("SYMMETRY"),
+    ]
+    num = make_numerics(True, **{"cfl": 10.0, **num_kw})
+    return Case("ramp_sa", mesh, num, bcs, rho, (u, 0.0, 0.0), p, default_linparams(),
+                info=dict(mach=mach, re=re, T=T, turbulence="SA"))
+
+
+def plate_sa(ni=256, nj=128, nk=64, re=5.0e6, **kw):
+    """The laminar plate's mesh and conditions at Re_L = 5e6 with the
+    Spalart-Allmaras module."""
+    c = plate(ni=ni, nj=nj, nk=nk, re=re, **kw)
+    c.name = "plate_sa"
+    c.info["turbulence"] = "SA"
+    return c
+
+
+def bump_sa(ni=150, nj=150, nk=150, **kw):
+    """The transonic tetrahedral bump (Mach 0.8, Re 1e6) with the
+    Spalart-Allmaras module."""
+    c = bump(ni=ni, nj=nj, nk=nk, **kw)
+    c.name = "bump_sa"
+    c.info["turbulence"] = "SA"
+    return c
+
+
+def turbulence(case, update=None, nu_inf_ratio=3.0):
+    """cfdg.Turbulence block of an SA case (freestream = the case's state)."""
+    from . import TURB_POINT_IMPLICIT, Turbulence
+    return Turbulence.make(rho_inf=case.rho, p_inf=case.p, update=TURB_POINT_IMPLICIT if update is None else update,
+                           nu_inf_ratio=nu_inf_ratio)
+
+
+CASES = {"ramp": ramp, "plate": plate, "bump": bump, "ramp_sa": ramp_sa, "plate_sa": plate_sa, "bump_sa": bump_sa}

@@ -250,8 +250,8 @@ KVecs<float> kvecs32(cfdg_ctx& c) {
 }
 
 template <class T>
-void spmv_stored(cfdg_ctx& c, const T* x, T* y) {
-    cfdg::launch_spmv<T>(c.bargs(), c.values.get(), c.storage, x, y, c.stream);
+void spmv_stored(cfdg_ctx& c, const T* x, T* y, const double* gate = nullptr) {
+    cfdg::launch_spmv<T>(c.bargs(), c.values.get(), c.storage, x, y, c.stream, gate);
 }
 
 // z = M^-1 r: z = 0, forward groups, backward groups (preconditioners.cpp:171-194).
@@ -259,7 +259,7 @@ void spmv_stored(cfdg_ctx& c, const T* x, T* y) {
 // rows read only neighbours that the forward sweep already left final, so their
 // backward update equals their forward one bit for bit and is skipped.
 template <class T>
-void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z) {
+void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr) {
     const long long n5 = 5LL * c.nc;
     if (P.kind == CFDG_PRECOND_NONE) {
         hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(T), hipMemcpyDeviceToDevice, c.stream),
@@ -274,7 +274,7 @@ void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z) {
         const auto& off = c.h_gofs[gi];
         for (std::size_t k = (sweep == 1 ? 1 : 0); k + 1 < off.size(); ++k)
             cfdg::launch_sgs_group<T>(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.storage,
-                                      c.lu.get(), c.piv.get(), r, z, c.stream);
+                                      c.lu.get(), c.piv.get(), r, z, c.stream, gate);
     }
 }
 
@@ -340,72 +340,75 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
     if (res.status == CFDG_MAX_ITERATIONS) {
         hip_check(hipMemcpyAsync(w.r0, w.r, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
         hip_check(hipMemsetAsync(w.v, 0, bytes, c.stream), "memset");
-        double rho = rnorm * rnorm, rho_old = 1.0, alpha = 1.0, omega = 1.0;
+        // The recurrence runs on the device: scalars in the state block ks,
+        // iterations enqueued in batches and the state read back once per
+        // batch; iterations past convergence or breakdown are no-ops.
+        double* ks = c.ks.get();
+        double* h = c.h_ks;
+        for (int k = 0; k < cfdg::kKsCount; ++k) h[k] = 0.0;
+        h[cfdg::kKsMaxIt] = static_cast<double>(lp.max_iter);
+        h[cfdg::kKsTol] = tol;
+        h[cfdg::kKsRho] = rnorm * rnorm;
+        h[cfdg::kKsRhoOld] = 1.0;
+        h[cfdg::kKsAlpha] = 1.0;
+        h[cfdg::kKsOmega] = 1.0;
+        h[cfdg::kKsRnorm] = rnorm;
+        c.h2d(ks, h, cfdg::kKsCount);
+        const int slot = (sizeof(T) == sizeof(double)) ? 0 : 1;
+        int batch = std::max(1, c.last_iters[slot] - 1);
+        long long enq = 0;
+        double* rr = c.dots.get() + 5;
         while (true) {
-            if (it >= lp.max_iter) {
-                res.status = CFDG_MAX_ITERATIONS;
-                break;
-            }
-            ++it;
-            if (it == 1) {
-                hip_check(hipMemcpyAsync(w.p, w.r, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
-            } else {
-                const double beta = (rho / rho_old) * (alpha / omega);
-                if (!std::isfinite(beta)) {
-                    res.status = CFDG_DIVERGED;
-                    break;
+            for (int b = 0; b < batch && enq <= lp.max_iter; ++b) {
+                ++enq;
+                cfdg::launch_bicg_begin(ks, c.stream);
+                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, c.stream);
+                precond_apply<T>(c, P, w.p, w.ph, ks);
+                spmv_stored<T>(c, w.ph, w.v, ks);
+                {
+                    cfdg::DotPairsT<T> pr{};
+                    pr.a[0] = w.r0;
+                    pr.b[0] = w.v;
+                    cfdg::launch_dots(N, 1, pr, c.partial.get(), c.dots.get(), c.stream, ks);
                 }
-                cfdg::launch_update_p<T>(N, beta, -beta * omega, w.r, w.v, w.p, c.stream);
-            }
-            precond_apply<T>(c, P, w.p, w.ph);
-            spmv_stored<T>(c, w.ph, w.v);
-            const double gamma = dot_device<T>(c, w.r0, w.v);
-            if (!std::isfinite(gamma) || gamma == 0.0) {
-                res.status = CFDG_DIVERGED;
-                break;
-            }
-            alpha = rho / gamma;
-            cfdg::launch_axpy_copy<T>(N, w.r, -alpha, w.v, w.s, c.stream);
-            precond_apply<T>(c, P, w.s, w.sh);
-            spmv_stored<T>(c, w.sh, w.t);
-            {
-                cfdg::DotPairsT<T> pr{};
-                const T* A[5] = {w.t, w.t, w.r0, w.r0, w.s};
-                const T* B[5] = {w.t, w.s, w.t, w.s, w.s};
-                for (int j = 0; j < 5; ++j) {
-                    pr.a[j] = A[j];
-                    pr.b[j] = B[j];
+                cfdg::launch_bicg_alpha(ks, c.dots.get(), c.stream);
+                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, c.stream);
+                precond_apply<T>(c, P, w.s, w.sh, ks);
+                spmv_stored<T>(c, w.sh, w.t, ks);
+                {
+                    cfdg::DotPairsT<T> pr{};
+                    const T* A[5] = {w.t, w.t, w.r0, w.r0, w.s};
+                    const T* B[5] = {w.t, w.s, w.t, w.s, w.s};
+                    for (int j = 0; j < 5; ++j) {
+                        pr.a[j] = A[j];
+                        pr.b[j] = B[j];
+                    }
+                    cfdg::launch_dots(N, 5, pr, c.partial.get(), c.dots.get(), c.stream, ks);
                 }
-                cfdg::launch_dots(N, 5, pr, c.partial.get(), c.dots.get(), c.stream);
-                fetch_dots(c, 5);
+                cfdg::launch_bicg_omega(ks, c.dots.get(), c.stream);
+                cfdg::launch_update_xr_g<T>(N, ks, w.ph, w.sh, w.s, w.t, x, w.r, c.stream);
+                if (enq % 32 == 0) {  // true residual norm every 32 iterations
+                    cfdg::DotPairsT<T> pr{};
+                    pr.a[0] = w.r;
+                    pr.b[0] = w.r;
+                    cfdg::launch_dots(N, 1, pr, c.partial.get(), rr, c.stream, ks);
+                    cfdg::launch_bicg_post(ks, rr, c.stream);
+                } else {
+                    cfdg::launch_bicg_post(ks, nullptr, c.stream);
+                }
             }
-            const double tt = c.h_dots[0], ts = c.h_dots[1], r0t = c.h_dots[2], r0s = c.h_dots[3], ss = c.h_dots[4];
-            if (!std::isfinite(ts) || !std::isfinite(tt) || tt == 0.0) {
-                res.status = CFDG_DIVERGED;
-                break;
-            }
-            const double omega_new = ts / tt;
-            cfdg::launch_update_xr<T>(N, alpha, omega_new, w.ph, w.sh, w.s, w.t, x, w.r, c.stream);
-            const double rn2 = ss - 2.0 * omega_new * ts + omega_new * omega_new * tt;
-            rnorm = std::sqrt(std::max(rn2, 0.0));
-            const double rho_new = r0s - omega_new * r0t;
-            if (!std::isfinite(rnorm) || !std::isfinite(omega_new) || omega_new == 0.0) {
-                res.status = CFDG_DIVERGED;
-                break;
-            }
-            if (it % 32 == 0) rnorm = std::sqrt(dot_device<T>(c, w.r, w.r));
-            if (rnorm <= tol) {
-                res.status = CFDG_CONVERGED;
-                break;
-            }
-            if (!std::isfinite(rho_new) || rho_new == 0.0) {
-                res.status = CFDG_DIVERGED;
-                break;
-            }
-            rho_old = rho;
-            rho = rho_new;
-            omega = omega_new;
+            c.check_launch();
+            c.d2h(h, ks, cfdg::kKsCount);
+            c.sync();
+            if (h[cfdg::kKsStatus] != 0.0 || enq > lp.max_iter) break;
+            batch = 2;
         }
+        it = static_cast<int>(h[cfdg::kKsIt]);
+        rnorm = h[cfdg::kKsRnorm];
+        const double st = h[cfdg::kKsStatus];
+        res.status = (st == cfdg::kKsConverged) ? CFDG_CONVERGED
+                                                 : (st == cfdg::kKsDiverged ? CFDG_DIVERGED : CFDG_MAX_ITERATIONS);
+        c.last_iters[slot] = it;
     }
     c.check_launch();
     c.sync();
@@ -563,6 +566,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : c.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c.h_dots), 8 * sizeof(double)), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&c.h_ks), cfdg::kKsCount * sizeof(double)), "hipHostMalloc");
 
     c.num = *num;
     c.nc = m.n_cells;
@@ -735,6 +739,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     }
     c.partial.alloc(static_cast<std::size_t>(cfdg::kReduceBlocks) * 5);
     c.dots.alloc(8);
+    c.ks.alloc(cfdg::kKsCount);
     c.sync();
 }
 

@@ -118,6 +118,9 @@ struct cfdg_ctx {
     } f32;
     DBuf<double> partial, dots;
     double* h_dots = nullptr;  // pinned
+    DBuf<double> ks;           // device-driven BiCGSTAB state block (kernels_linalg.hpp)
+    double* h_ks = nullptr;    // pinned copy
+    int last_iters[2] = {4, 4};  // iterations of the last FP64 / FP32 solve: the first batch size
     // learned warm start (gnn.hpp)
     std::uint64_t uid = 0;              // identity of this context's graph in recorded samples
     bool have_ref = false;              // freestream of the feature normalisation
@@ -243,6 +246,7 @@ struct cfdg_ctx {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_dots) (void)hipHostFree(h_dots);
+        if (h_ks) (void)hipHostFree(h_ks);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

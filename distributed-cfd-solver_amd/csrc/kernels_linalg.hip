@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
 // acc_ii = sum_k (sum_jj a_k[ii][jj] x_col(k)[jj]) in column order.
 template <class TS, class TV>
 __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, const int* cols, const TS* values,
-                                                 const TV* x, TV* y) {
+                                                 const TV* x, TV* y, const double* gate) {
+    if (gate && gate[kKsStatus] != 0.0) return;
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long row = t / 5;
     const int ii = static_cast<int>(t - row * 5);
@@ -119,7 +120,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, cons
 template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
                                                       const int* diag, const TS* values, const TL* lu,
-                                                      const std::uint8_t* piv, const TV* r, TV* z) {
+                                                      const std::uint8_t* piv, const TV* r, TV* z, const double* gate) {
+    if (gate && gate[kKsStatus] != 0.0) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int i = rows[t];
@@ -204,6 +206,131 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(long long n, double alpha,
     r[i] = rv;
 }
 
+// --- device-driven BiCGSTAB --------------------------------------------------
+// The scalars of the recurrence live in the state block ks (kernels_linalg.hpp)
+// and one-thread kernels update them from the dot products, so a batch of
+// iterations is enqueued without host round trips.  Every kernel of an
+// iteration returns at once when ks[kKsStatus] is no longer "running", so the
+// iterations enqueued after convergence or breakdown change nothing and the
+// result is the one the per-iteration host loop of bicgstab.cpp:42-201 gives.
+
+// Start of an iteration (bicgstab.cpp:97-109): iteration limit, it += 1,
+// beta = (rho / rho_old)(alpha / omega) from the second iteration on.
+__global__ void k_bicg_begin(double* ks) {
+    if (ks[kKsStatus] != 0.0) return;
+    if (ks[kKsIt] >= ks[kKsMaxIt]) {
+        ks[kKsStatus] = kKsMaxIter;
+        return;
+    }
+    ks[kKsIt] += 1.0;
+    ks[kKsFirst] = (ks[kKsIt] == 1.0) ? 1.0 : 0.0;
+    if (ks[kKsFirst] == 0.0) {
+        const double beta = (ks[kKsRho] / ks[kKsRhoOld]) * (ks[kKsAlpha] / ks[kKsOmega]);
+        if (!isfinite(beta)) {
+            ks[kKsStatus] = kKsDiverged;
+            return;
+        }
+        ks[kKsBeta] = beta;
+        ks[kKsNbo] = -beta * ks[kKsOmega];
+    }
+}
+
+// gamma = (r0, v); alpha = rho / gamma (bicgstab.cpp:113-120)
+__global__ void k_bicg_alpha(double* ks, const double* dots) {
+    if (ks[kKsStatus] != 0.0) return;
+    const double gamma = dots[0];
+    if (!isfinite(gamma) || gamma == 0.0) {
+        ks[kKsStatus] = kKsDiverged;
+        return;
+    }
+    ks[kKsAlpha] = ks[kKsRho] / gamma;
+}
+
+// omega = (t, s) / (t, t); the recurrence norm of the new r and rho_new
+// (bicgstab.cpp:126-160); dots = (t,t), (t,s), (r0,t), (r0,s), (s,s)
+__global__ void k_bicg_omega(double* ks, const double* dots) {
+    if (ks[kKsStatus] != 0.0) return;
+    const double tt = dots[0], ts = dots[1], r0t = dots[2], r0s = dots[3], ss = dots[4];
+    if (!isfinite(ts) || !isfinite(tt) || tt == 0.0) {
+        ks[kKsStatus] = kKsDiverged;
+        return;
+    }
+    const double omega_new = ts / tt;
+    const double rn2 = ss - 2.0 * omega_new * ts + omega_new * omega_new * tt;
+    ks[kKsOmegaNew] = omega_new;
+    ks[kKsRnNew] = sqrt(rn2 > 0.0 ? rn2 : 0.0);
+    ks[kKsRhoNew] = r0s - omega_new * r0t;
+}
+
+// After x and r are updated (bicgstab.cpp:160-195): breakdown checks, the
+// true residual norm when rr is given (every 32nd iteration), convergence.
+__global__ void k_bicg_post(double* ks, const double* rr) {
+    if (ks[kKsStatus] != 0.0) return;
+    double rnorm = ks[kKsRnNew];
+    const double omega_new = ks[kKsOmegaNew];
+    ks[kKsRnorm] = rnorm;
+    if (!isfinite(rnorm) || !isfinite(omega_new) || omega_new == 0.0) {
+        ks[kKsStatus] = kKsDiverged;
+        return;
+    }
+    if (rr) rnorm = sqrt(rr[0]);
+    ks[kKsRnorm] = rnorm;
+    if (rnorm <= ks[kKsTol]) {
+        ks[kKsStatus] = kKsConverged;
+        return;
+    }
+    const double rho_new = ks[kKsRhoNew];
+    if (!isfinite(rho_new) || rho_new == 0.0) {
+        ks[kKsStatus] = kKsDiverged;
+        return;
+    }
+    ks[kKsRhoOld] = ks[kKsRho];
+    ks[kKsRho] = rho_new;
+    ks[kKsOmega] = omega_new;
+}
+
+// p = r on the first iteration, else p = ((p beta) + (-beta omega) v) + r
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p) {
+    if (ks[kKsStatus] != 0.0) return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (ks[kKsFirst] != 0.0) {
+        p[i] = r[i];
+        return;
+    }
+    T t = p[i] * static_cast<T>(ks[kKsBeta]);
+    t += static_cast<T>(ks[kKsNbo]) * v[i];
+    t += static_cast<T>(1.0) * r[i];
+    p[i] = t;
+}
+
+// s = r + (-alpha) v
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv) {
+    if (ks[kKsStatus] != 0.0) return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sv[i] = r[i] + static_cast<T>(-ks[kKsAlpha]) * v[i];
+}
+
+// x = (x + alpha ph) + omega sh;  r = s + (-omega) t
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update_xr_g(long long n, const double* ks, const T* ph, const T* sh,
+                                                        const T* sv, const T* t, T* x, T* r) {
+    if (ks[kKsStatus] != 0.0) return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = ks[kKsAlpha], omega = ks[kKsOmegaNew];
+    T xv = x[i];
+    xv += static_cast<T>(alpha) * ph[i];
+    xv += static_cast<T>(omega) * sh[i];
+    x[i] = xv;
+    T rv = sv[i];
+    rv += static_cast<T>(-omega) * t[i];
+    r[i] = rv;
+}
+
 // dst = src converted (defect correction: FP64 residual -> FP32 inner right-hand side)
 template <class TD, class TS>
 __global__ void __launch_bounds__(kBlock) k_cast(long long n, const TS* src, TD* dst) {
@@ -241,7 +368,9 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 
 // Up to five dot products of flat vectors in one pass; partial[b * 5 + j].
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_dots(long long n, int ndot, DotPairsT<T> pr, double* partial) {
+__global__ void __launch_bounds__(kBlock) k_dots(long long n, int ndot, DotPairsT<T> pr, double* partial,
+                                                 const double* gate) {
+    if (gate && gate[kKsStatus] != 0.0) return;
     __shared__ double lds[kBlock / 64];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
@@ -276,7 +405,8 @@ __global__ void __launch_bounds__(kBlock) k_res_sq(int nc, int ld, const double*
 // Adds nblocks partial 5-vectors: thread t sums blocks t, t + 256, ... in
 // order, then the block tree adds the 256 thread sums in a fixed order.
 __global__ void __launch_bounds__(kBlock) k_reduce_partials(int nblocks, int ncomp, const double* partial,
-                                                            double* out) {
+                                                            double* out, const double* gate) {
+    if (gate && gate[kKsStatus] != 0.0) return;
     __shared__ double lds[kBlock / 64];
     for (int j = 0; j < ncomp; ++j) {
         double s = 0.0;
@@ -326,47 +456,49 @@ void launch_lu_factor(int n, const int* diag, const void* values, int storage, v
 }
 
 template <class TV>
-void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s) {
+void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s,
+                 const double* gate) {
     if (b.n <= 0) return;
     const dim3 g = grid_for(5LL * b.n), blk(kBlock);
     if (storage == kStoreF32)
         hipLaunchKernelGGL((k_spmv<float, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols,
-                           static_cast<const float*>(values), x, y);
+                           static_cast<const float*>(values), x, y, gate);
     else if (storage == kStoreBF16)
         hipLaunchKernelGGL((k_spmv<bf16, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols, static_cast<const bf16*>(values),
-                           x, y);
+                           x, y, gate);
     else
         hipLaunchKernelGGL((k_spmv<double, TV>), g, blk, 0, s, b.n, b.row_ptr, b.cols,
-                           static_cast<const double*>(values), x, y);
+                           static_cast<const double*>(values), x, y, gate);
 }
-template void launch_spmv<double>(const BsrArgs&, const void*, int, const double*, double*, hipStream_t);
-template void launch_spmv<float>(const BsrArgs&, const void*, int, const float*, float*, hipStream_t);
+template void launch_spmv<double>(const BsrArgs&, const void*, int, const double*, double*, hipStream_t,
+                                  const double*);
+template void launch_spmv<float>(const BsrArgs&, const void*, int, const float*, float*, hipStream_t, const double*);
 
 template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
-                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s) {
+                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate) {
     if (count <= 0) return;
     const dim3 g = grid_for(count), blk(kBlock);
     if constexpr (sizeof(TV) == sizeof(double)) {
         if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
         hipLaunchKernelGGL((k_sgs_group<double, double, double>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
-                           static_cast<const double*>(values), static_cast<const double*>(lu), piv, r, z);
+                           static_cast<const double*>(values), static_cast<const double*>(lu), piv, r, z, gate);
     } else {
         if (storage == kStoreF32)
             hipLaunchKernelGGL((k_sgs_group<float, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
-                               b.diag, static_cast<const float*>(values), static_cast<const float*>(lu), piv, r, z);
+                               b.diag, static_cast<const float*>(values), static_cast<const float*>(lu), piv, r, z, gate);
         else if (storage == kStoreBF16)
             hipLaunchKernelGGL((k_sgs_group<bf16, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
-                               static_cast<const bf16*>(values), static_cast<const float*>(lu), piv, r, z);
+                               static_cast<const bf16*>(values), static_cast<const float*>(lu), piv, r, z, gate);
         else
             hipLaunchKernelGGL((k_sgs_group<double, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
-                               b.diag, static_cast<const double*>(values), static_cast<const float*>(lu), piv, r, z);
+                               b.diag, static_cast<const double*>(values), static_cast<const float*>(lu), piv, r, z, gate);
     }
 }
 template void launch_sgs_group<double>(const int*, int, const BsrArgs&, const void*, int, const void*,
-                                       const std::uint8_t*, const double*, double*, hipStream_t);
+                                       const std::uint8_t*, const double*, double*, hipStream_t, const double*);
 template void launch_sgs_group<float>(const int*, int, const BsrArgs&, const void*, int, const void*,
-                                      const std::uint8_t*, const float*, float*, hipStream_t);
+                                      const std::uint8_t*, const float*, float*, hipStream_t, const double*);
 
 template <class T>
 void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s) {
@@ -413,19 +545,58 @@ int reduction_blocks(long long n) {
 }
 
 template <class T>
-void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s) {
+void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s,
+                 const double* gate) {
     const int nb = reduction_blocks(n);
-    hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out);
+    hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial, gate);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out, gate);
 }
 
-template void launch_dots<double>(long long, int, const DotPairsT<double>&, double*, double*, hipStream_t);
-template void launch_dots<float>(long long, int, const DotPairsT<float>&, double*, double*, hipStream_t);
+template void launch_dots<double>(long long, int, const DotPairsT<double>&, double*, double*, hipStream_t,
+                                  const double*);
+template void launch_dots<float>(long long, int, const DotPairsT<float>&, double*, double*, hipStream_t,
+                                 const double*);
+
+void launch_bicg_begin(double* ks, hipStream_t s) { hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(1), 0, s, ks); }
+void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1), 0, s, ks, dots);
+}
+void launch_bicg_omega(double* ks, const double* dots, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1), 0, s, ks, dots);
+}
+void launch_bicg_post(double* ks, const double* rr, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_post, dim3(1), dim3(1), 0, s, ks, rr);
+}
+
+template <class T>
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_update_p_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, p);
+}
+template void launch_update_p_g<double>(long long, const double*, const double*, const double*, double*, hipStream_t);
+template void launch_update_p_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
+
+template <class T>
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy_alpha_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, sv);
+}
+template void launch_axpy_alpha_g<double>(long long, const double*, const double*, const double*, double*,
+                                          hipStream_t);
+template void launch_axpy_alpha_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
+
+template <class T>
+void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,
+                        hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_update_xr_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, x, r);
+}
+template void launch_update_xr_g<double>(long long, const double*, const double*, const double*, const double*,
+                                         const double*, double*, double*, hipStream_t);
+template void launch_update_xr_g<float>(long long, const double*, const float*, const float*, const float*,
+                                        const float*, float*, float*, hipStream_t);
 
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s) {
     const int nb = reduction_blocks(nc);
     hipLaunchKernelGGL(k_res_sq, dim3(nb), dim3(kBlock), 0, s, nc, ld, res, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, 5, partial, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, 5, partial, out, nullptr);
 }
 
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s) {

@@ -15,6 +15,15 @@ namespace cfdg {
 
 constexpr int kReduceBlocks = 1024;  // partial sums per reduction (fixed -> deterministic)
 
+// State block of the device-driven BiCGSTAB (doubles).  kKsStatus is 0 while
+// the solve runs, then one of the codes below; every gated kernel returns at
+// once when it is not 0.
+enum : int {
+    kKsStatus = 0, kKsIt, kKsMaxIt, kKsTol, kKsRho, kKsRhoOld, kKsAlpha, kKsOmega, kKsBeta, kKsNbo, kKsFirst,
+    kKsOmegaNew, kKsRnNew, kKsRhoNew, kKsRnorm, kKsCount = 16
+};
+constexpr double kKsConverged = 1.0, kKsDiverged = 2.0, kKsMaxIter = 3.0;
+
 template <class T>
 struct DotPairsT {
     const T* a[5];
@@ -26,12 +35,14 @@ using DotPairs = DotPairsT<double>;
 // and kept in FP64 (lu_f32 = 0, FP64 blocks only) or FP32 (lu_f32 = 1).
 void launch_lu_factor(int n, const int* diag, const void* values, int storage, void* lu, int lu_f32,
                       std::uint8_t* piv, int* singular, hipStream_t s);
+// gate: optional BiCGSTAB state block; the kernel does nothing once its status is set
 template <class TV>
-void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s);
+void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s,
+                 const double* gate = nullptr);
 // TV = double: FP64 blocks and FP64 LU.  TV = float: any storage, FP32 LU.
 template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
-                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s);
+                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate = nullptr);
 template <class T>
 void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s);
 template <class T>
@@ -45,7 +56,20 @@ void launch_add_f32(long long n, const float* e, double* x, hipStream_t s);
 int reduction_blocks(long long n);
 // out[0..ndot) = dot products accumulated in FP64; partial needs kReduceBlocks * 5 doubles
 template <class T>
-void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s);
+void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s,
+                 const double* gate = nullptr);
+// device-driven BiCGSTAB: one-thread scalar steps and gated vector updates
+void launch_bicg_begin(double* ks, hipStream_t s);
+void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s);
+void launch_bicg_omega(double* ks, const double* dots, hipStream_t s);
+void launch_bicg_post(double* ks, const double* rr, hipStream_t s);
+template <class T>
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s);
+template <class T>
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s);
+template <class T>
+void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,
+                        hipStream_t s);
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s);
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s);
 void launch_state_update(int nc, int ld, const double* du, double* U, hipStream_t s);
