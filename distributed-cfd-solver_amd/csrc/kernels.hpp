@@ -57,7 +57,11 @@ struct CellRef {
     const double* b;
     long long ld;
     int c;
-    __host__ __device__ __forceinline__ double operator[](int k) const { return b[plane_at(k, ld, c)]; }
+    // uniform plane-pair base + 32-bit per-lane offset (scalar base address + vector offset loads)
+    __host__ __device__ __forceinline__ double operator[](int k) const {
+        const double* base = b + static_cast<long long>(k >> 1) * 2 * ld;
+        return base[2u * static_cast<unsigned>(c) + static_cast<unsigned>(k & 1)];
+    }
 };
 struct CellRefW {
     double* b;
@@ -70,6 +74,18 @@ struct FaceRef {
     long long ld;
     int f;
     __host__ __device__ __forceinline__ double operator[](int k) const { return b[plane_at(k, ld, f)]; }
+};
+
+// Face planes read once per launch (the tile face-list stream): non-temporal
+// loads, so the stream does not evict the cell planes that faces share.
+struct FaceRefNT {
+    const double* b;
+    long long ld;
+    int f;
+    __device__ __forceinline__ double operator[](int k) const {
+        const double* base = b + static_cast<long long>(k >> 1) * 2 * ld;
+        return __builtin_nontemporal_load(base + (2u * static_cast<unsigned>(f) + static_cast<unsigned>(k & 1)));
+    }
 };
 
 struct MeshArgs {

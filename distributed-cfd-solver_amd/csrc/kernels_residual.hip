@@ -459,9 +459,11 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
     const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
     for (int i = threadIdx.x; i < nfl; i += kTileCells) {
         double F[5], lam_f;
-        const int fc = m.tile_faces[fb + i];
-        const int2 cc = m.tcells[fb + i];
-        face_flux<kTurb>(f, nm, FaceRef{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
+        const int fc = __builtin_nontemporal_load(m.tile_faces + fb + i);
+        const int cx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + fb + i));
+        const int cy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + fb + i) + 1);
+        const int2 cc = make_int2(cx, cy);
+        face_flux<kTurb>(f, nm, FaceRefNT{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
         if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
         for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
         sF[5][i] = lam_f;
@@ -484,8 +486,8 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
             lam += sF[5][k];
         }
     }
-    for (int v = 0; v < 5; ++v) f.res[static_cast<long long>(v) * m.nc + c] = acc[v];
-    f.lam[c] = lam;
+    for (int v = 0; v < 5; ++v) __builtin_nontemporal_store(acc[v], f.res + static_cast<long long>(v) * m.nc + c);
+    __builtin_nontemporal_store(lam, f.lam + c);
 }
 
 // Implicit local time step and diagonal shift (solver.hpp:256-287)
