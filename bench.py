@@ -130,13 +130,27 @@ def main():
 
     # one full implicit step (residual, dt, Jacobian, BiCGSTAB + multicolour SGS, update)
     lp = cases.default_linparams()
-    imp = []
-    for _ in range(a.implicit_steps):
-        t1 = time.perf_counter()
-        st = solver.implicit_step(lp)
-        imp.append({"ms": 1e3 * (time.perf_counter() - t1), "linear_iterations": st.linear.iterations,
-                    "linear_status": st.linear.status,
-                    "jacobian_ms": float(solver.stage_times()[4]), "linear_ms": float(solver.stage_times()[5])})
+
+    def implicit_steps():
+        rows = []
+        for _ in range(a.implicit_steps):
+            t1 = time.perf_counter()
+            st = solver.implicit_step(lp)
+            rows.append({"ms": 1e3 * (time.perf_counter() - t1), "linear_iterations": st.linear.iterations,
+                         "linear_status": st.linear.status, "jacobian_ms": float(solver.stage_times()[4]),
+                         "linear_ms": float(solver.stage_times()[5])})
+        return rows
+
+    imp = implicit_steps()
+    # the same steps from the same state in the two reduced-precision forms
+    imp_forms = {}
+    for name, prec in (("fp32_blocks_fp32_krylov", cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, 8, 0.1)),
+                       ("bf16_blocks_fp32_krylov", cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, 16, 0.1))):
+        solver.set_precision(prec)
+        solver.set_state(U)
+        imp_forms[name] = implicit_steps()
+    solver.set_precision(cfdg.Precision.make())
+    solver.set_state(U)
 
     cpu = None
     if a.cpu_baseline != "off":
@@ -178,6 +192,7 @@ def main():
                      "ghost_gradients": round(stage_ms[2], 4), "flux": round(stage_ms[3], 4)},
         "flux_kernel_mcells_s": round(nc / (flux_ms * 1e-3) / 1e6, 1),
         "implicit_step": imp,
+        "implicit_step_forms": imp_forms,
         "setup_s": {"mesh": round(t_mesh, 2), "context": round(t_setup, 2)},
     }
     if a.drop_size != "off":
@@ -194,7 +209,10 @@ def time_to_drop(a):
     node-LSQ gradient (the Venkatakrishnan limiter stalls this case near 1e-6),
     CFL 1000, from the freestream with seeded 2 % noise.  The network is
     trained first on 8 seeded 32x16x8 plates with Mach drawn from U(0.2, 0.5)
-    (20 recorded steps each); its training time is reported separately."""
+    (every 100th of 2000 steps recorded, so the samples span the whole
+    convergence history); its training time is reported separately.  The
+    previous-update guess alone (the network with all parameters zero) is
+    timed too, so the network's own share of the learned form shows."""
     import numpy as np
 
     import cfdg
@@ -233,6 +251,9 @@ def time_to_drop(a):
                        "CFL 1000, BiCGSTAB + multicolour SGS (rel 0.1), 2% seeded noise on the freestream"}
     out["plain_fp64"], U0 = run("plain")
     out["fp32_krylov"], U1 = run("fp32")
+    zero = cfdg.Gnn(seed=1)
+    zero.set_params(np.zeros(zero.num_params, np.float32))
+    out["fp32_krylov_previous_update"], _ = run("learned", zero)
     t0 = time.perf_counter()
     gnn = cfdg.Gnn(seed=11)
     rng = np.random.default_rng(2024)
@@ -240,9 +261,9 @@ def time_to_drop(a):
         case = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m, **kw)
         s = start(case, m)
         s.set_recording(True)
-        for it in range(40):
+        for it in range(2000):
             s.implicit_step(lp)
-            if it % 2 == 1:
+            if it % 100 == 99:
                 gnn.record(s)
         s.close()
     loss = gnn.train(epochs=30, lr=1e-3, seed=5)

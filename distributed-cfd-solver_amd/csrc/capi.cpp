@@ -685,6 +685,11 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
             put(cfdg::kRVol, r, m.cell_vol[o]);
         }
         c.cs.upload(cp, s);
+        // inverse least-squares normal matrices: geometry only, once per mesh
+        if (num->gradient == CFDG_LEAST_SQUARES_NODE) {
+            cfdg::launch_lsq_setup(c.margs(), c.fargs(), s);
+            c.check_launch();
+        }
         c.sync();
     }
     // BSR: the mesh-order pattern (bsr_helpers.hpp:16-59) and its device rows

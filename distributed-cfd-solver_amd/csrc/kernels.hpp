@@ -31,12 +31,13 @@ constexpr int kStoreF64 = 0, kStoreF32 = 1, kStoreBF16 = 2;
 // so that one 16-byte load brings two quantities of a cell.  Consecutive lanes
 // of a wave work on neighbouring cells of one tile, so every load of a wave
 // falls in a few cache lines.
-constexpr int kCellPlanes = 30;  // 29 used
+constexpr int kCellPlanes = 36;  // 35 used
 constexpr int kRQ = 0;     // p, u, v, w, T
 constexpr int kRC = 5;     // centroid x, y, z
 constexpr int kRPhi = 8;   // limiter of the 5 primitives
 constexpr int kRVol = 13;  // volume
 constexpr int kRG = 14;    // gradients, (3 v + d)
+constexpr int kRInv = 30;  // inverse LSQ normal matrix (00, 01, 02, 11, 12, 22), set-up time
 
 // Face data: kFacePlanes planes of MeshArgs::fld doubles in mesh face order
 // (unit normal, area, centroid, owner->neighbour centroid vector -- mirrored
@@ -180,6 +181,8 @@ __host__ __device__ __forceinline__ int xcd_block(int b, int nb) {
 void launch_primitives(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
 void launch_ghosts(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs, hipStream_t s);
 void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
+// inverse LSQ normal matrices (planes kRInv..), once the centroids and the vertex stencil are on the device
+void launch_lsq_setup(const MeshArgs& m, const FieldArgs& f, hipStream_t s);
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,
                             hipStream_t s);
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);

@@ -130,6 +130,56 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(MeshArgs m, FieldArgs f, NumA
     for (int v = 0; v < 5; ++v) rg[kRQ + v] = qg[v];
 }
 
+// Inverse of the weighted least-squares normal matrix of cell c over its
+// vertex-sharing stencil (LeastSquaresCellNodeGradient::setup,
+// gradient_operator.cpp:622-707): A = sum w dr dr^T + 1e-14 I with
+// w = 1/max(|dr|^2, 1e-30), inverted by the adjugate (zero when |det| <= 1e-14).
+// A is symmetric and so is its adjugate inverse entry for entry, so six values
+// (00, 01, 02, 11, 12, 22) hold it.
+__device__ __forceinline__ void lsq_inverse(const MeshArgs& m, const FieldArgs& f, int c, double out[6]) {
+    const CellRef rc = crec(f, c);
+    const double x0 = rc[kRC], y0 = rc[kRC + 1], z0 = rc[kRC + 2];
+    double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
+    for (int j = m.ccn_off[c]; j < m.ccn_off[c + 1]; ++j) {
+        const CellRef rn = crec(f, m.ccn[j]);
+        const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double w = 1.0 / dmax(r2, 1.0e-30);
+        A0 += w * dx * dx;
+        A1 += w * dx * dy;
+        A2 += w * dx * dz;
+        A4 += w * dy * dy;
+        A5 += w * dy * dz;
+        A8 += w * dz * dz;
+    }
+    const double A3 = A1, A6 = A2, A7 = A5;
+    A0 += 1.0e-14;
+    A4 += 1.0e-14;
+    A8 += 1.0e-14;
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    const double det = A0 * (A4 * A8 - A5 * A7) - A1 * (A3 * A8 - A5 * A6) + A2 * (A3 * A7 - A4 * A6);
+    if (!(fabs(det) <= 1.0e-14)) {
+        const double id = 1.0 / det;
+        out[0] = (A4 * A8 - A5 * A7) * id;
+        out[1] = (A2 * A7 - A1 * A8) * id;
+        out[2] = (A1 * A5 - A2 * A4) * id;
+        out[3] = (A0 * A8 - A2 * A6) * id;
+        out[4] = (A2 * A3 - A0 * A5) * id;
+        out[5] = (A0 * A4 - A1 * A3) * id;
+    }
+}
+
+// Set-up: the inverse normal matrix of every cell into planes kRInv..kRInv+5
+// (centroids only, so once per mesh).
+__global__ void __launch_bounds__(kBlock) k_lsq_setup(MeshArgs m, FieldArgs f) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    double inv[6];
+    lsq_inverse(m, f, c, inv);
+    const CellRefW w = crecw(f, c);
+    for (int k = 0; k < 6; ++k) w[kRInv + k] = inv[k];
+}
+
 // Cell gradient + MUSCL limiter of device cell c (gradient_operator.cpp:549-707,
 // :197-278; muscl.hpp:26-104).  `self`, nbn(j) and nbf(j) point at 8 doubles
 // laid out as the first half of a cell record (q, centroid) for the cell, its
@@ -146,53 +196,23 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
     for (int v = 0; v < 5; ++v) g[v][0] = g[v][1] = g[v][2] = 0.0;
 
     if (nm.gradient == 2) {
-        // weighted least squares over the vertex-sharing stencil: the normal
-        // matrix A = sum w dr dr^T + 1e-14 I (w = 1/|dr|^2) is rebuilt from the
-        // neighbour centroids, inverted by the adjugate, and the coefficients
-        // C_j = A^-1 w_j dr_j are applied on the fly -- the same arithmetic as
-        // the reference's set-up-time coefficients.
+        // weighted least squares over the vertex-sharing stencil with the
+        // inverse normal matrix of the cell (set-up time, lsq_inverse): the
+        // coefficients C_j = A^-1 w_j dr_j are applied on the fly
         const int jb = m.ccn_off[c], je = m.ccn_off[c + 1];
         if (je > jb) {
-            double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
-            for (int j = jb; j < je; ++j) {
-                const auto rn = nbn(j);
-                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double w = 1.0 / dmax(r2, 1.0e-30);
-                A0 += w * dx * dx;
-                A1 += w * dx * dy;
-                A2 += w * dx * dz;
-                A4 += w * dy * dy;
-                A5 += w * dy * dz;
-                A8 += w * dz * dz;
-            }
-            const double A3 = A1, A6 = A2, A7 = A5;
-            A0 += 1.0e-14;
-            A4 += 1.0e-14;
-            A8 += 1.0e-14;
-            double inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            const double det = A0 * (A4 * A8 - A5 * A7) - A1 * (A3 * A8 - A5 * A6) + A2 * (A3 * A7 - A4 * A6);
-            if (!(fabs(det) <= 1.0e-14)) {
-                const double id = 1.0 / det;
-                inv[0] = (A4 * A8 - A5 * A7) * id;
-                inv[1] = (A2 * A7 - A1 * A8) * id;
-                inv[2] = (A1 * A5 - A2 * A4) * id;
-                inv[3] = (A5 * A6 - A3 * A8) * id;
-                inv[4] = (A0 * A8 - A2 * A6) * id;
-                inv[5] = (A2 * A3 - A0 * A5) * id;
-                inv[6] = (A3 * A7 - A4 * A6) * id;
-                inv[7] = (A1 * A6 - A0 * A7) * id;
-                inv[8] = (A0 * A4 - A1 * A3) * id;
-            }
+            const CellRef ri{f.cs, f.ld, c};
+            const double i0 = ri[kRInv], i1 = ri[kRInv + 1], i2 = ri[kRInv + 2], i4 = ri[kRInv + 3],
+                         i5 = ri[kRInv + 4], i8 = ri[kRInv + 5];
             for (int j = jb; j < je; ++j) {
                 const auto rn = nbn(j);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
                 const double rx = w * dx, ry = w * dy, rz = w * dz;
-                const double cx = inv[0] * rx + inv[1] * ry + inv[2] * rz;
-                const double cy = inv[3] * rx + inv[4] * ry + inv[5] * rz;
-                const double cz = inv[6] * rx + inv[7] * ry + inv[8] * rz;
+                const double cx = i0 * rx + i1 * ry + i2 * rz;
+                const double cy = i1 * rx + i4 * ry + i5 * rz;
+                const double cz = i2 * rx + i5 * ry + i8 * rz;
                 for (int v = 0; v < 5; ++v) {
                     const double d = rn[kRQ + v] - qc[v];
                     g[v][0] += cx * d;
@@ -622,6 +642,10 @@ void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
                            nm);
     else
         hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
+}
+
+void launch_lsq_setup(const MeshArgs& m, const FieldArgs& f, hipStream_t s) {
+    if (m.nc > 0) hipLaunchKernelGGL(k_lsq_setup, grid_for(m.nc), dim3(kBlock), 0, s, m, f);
 }
 
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,

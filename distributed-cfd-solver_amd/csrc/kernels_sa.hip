@@ -4,8 +4,8 @@
 //                  ghost values of the boundary faces (one thread per cell or
 //                  boundary face)
 //   k_sa_gradient  nu_tilde gradient with the configured method, one thread
-//                  per cell (node least squares rebuilt from the centroids as
-//                  the mean-flow gradient kernel does, or Green-Gauss)
+//                  per cell (node least squares with the cell's inverse normal
+//                  matrix, as the mean-flow gradient kernel, or Green-Gauss)
 //   k_sa_residual  per cell, over its faces in ascending face index: upwind
 //                  convection on the recorded mean mass flux, skew-corrected
 //                  diffusion, the interior faces' spectral radius added to
@@ -80,36 +80,9 @@ __global__ void __launch_bounds__(kBlock) k_sa_gradient(MeshArgs m, FieldArgs f,
         const double x0 = rc[kRC], y0 = rc[kRC + 1], z0 = rc[kRC + 2];
         const int jb = m.ccn_off[c], je = m.ccn_off[c + 1];
         if (je > jb) {
-            double A0 = 0.0, A1 = 0.0, A2 = 0.0, A4 = 0.0, A5 = 0.0, A8 = 0.0;
-            for (int j = jb; j < je; ++j) {
-                const CellRef rn = crec(f, m.ccn[j]);
-                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
-                const double w = 1.0 / dmax(dx * dx + dy * dy + dz * dz, 1.0e-30);
-                A0 += w * dx * dx;
-                A1 += w * dx * dy;
-                A2 += w * dx * dz;
-                A4 += w * dy * dy;
-                A5 += w * dy * dz;
-                A8 += w * dz * dz;
-            }
-            const double A3 = A1, A6 = A2, A7 = A5;
-            A0 += 1.0e-14;
-            A4 += 1.0e-14;
-            A8 += 1.0e-14;
-            double inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            const double det = A0 * (A4 * A8 - A5 * A7) - A1 * (A3 * A8 - A5 * A6) + A2 * (A3 * A7 - A4 * A6);
-            if (!(fabs(det) <= 1.0e-14)) {
-                const double id = 1.0 / det;
-                inv[0] = (A4 * A8 - A5 * A7) * id;
-                inv[1] = (A2 * A7 - A1 * A8) * id;
-                inv[2] = (A1 * A5 - A2 * A4) * id;
-                inv[3] = (A5 * A6 - A3 * A8) * id;
-                inv[4] = (A0 * A8 - A2 * A6) * id;
-                inv[5] = (A2 * A3 - A0 * A5) * id;
-                inv[6] = (A3 * A7 - A4 * A6) * id;
-                inv[7] = (A1 * A6 - A0 * A7) * id;
-                inv[8] = (A0 * A4 - A1 * A3) * id;
-            }
+            // inverse normal matrix of the cell, set up with the mesh (planes kRInv..)
+            const double i0 = rc[kRInv], i1 = rc[kRInv + 1], i2 = rc[kRInv + 2], i4 = rc[kRInv + 3],
+                         i5 = rc[kRInv + 4], i8 = rc[kRInv + 5];
             for (int j = jb; j < je; ++j) {
                 const int n = m.ccn[j];
                 const CellRef rn = crec(f, n);
@@ -117,9 +90,9 @@ __global__ void __launch_bounds__(kBlock) k_sa_gradient(MeshArgs m, FieldArgs f,
                 const double w = 1.0 / dmax(dx * dx + dy * dy + dz * dz, 1.0e-30);
                 const double rx = w * dx, ry = w * dy, rz = w * dz;
                 const double d = sa.nu[n] - s0;
-                g0 += (inv[0] * rx + inv[1] * ry + inv[2] * rz) * d;
-                g1 += (inv[3] * rx + inv[4] * ry + inv[5] * rz) * d;
-                g2 += (inv[6] * rx + inv[7] * ry + inv[8] * rz) * d;
+                g0 += (i0 * rx + i1 * ry + i2 * rz) * d;
+                g1 += (i1 * rx + i4 * ry + i5 * rz) * d;
+                g2 += (i2 * rx + i5 * ry + i8 * rz) * d;
             }
         }
     } else {
