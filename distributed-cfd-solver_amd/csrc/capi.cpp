@@ -173,10 +173,11 @@ void eval_residual(cfdg_ctx& c) {
     hip_check(hipEventRecord(c.ev[1], c.stream), "event");
     const bool need_grad = c.nargs.muscl || c.nargs.viscous;
     if (need_grad) {
-        cfdg::launch_gradient_limiter(m, f, c.nargs, c.stream);
+        const cfdg::SaArgs sa = c.sargs();
+        const bool sa_done = cfdg::launch_gradient_limiter(m, f, c.nargs, c.stream, c.sa_on() ? &sa : nullptr);
         hip_check(hipEventRecord(c.ev[2], c.stream), "event");
         cfdg::launch_ghost_gradients(m, f, c.nargs, c.bcs, c.stream);
-        if (c.sa_on()) cfdg::launch_sa_gradient(m, f, c.nargs, c.sargs(), c.stream);
+        if (c.sa_on() && !sa_done) cfdg::launch_sa_gradient(m, f, c.nargs, sa, c.stream);
     } else {
         hip_check(hipEventRecord(c.ev[2], c.stream), "event");
     }
@@ -278,6 +279,10 @@ void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double
     }
 }
 
+// Diagonal-block LU of the SGS preconditioner (preconditioners.cpp:62-93).
+// The singular-pivot flag is read with the solve's result (precond_check):
+// a zero pivot makes the solve produce non-finite values, and the error names
+// the row either way.
 void precond_setup(cfdg_ctx& c, const Precond& P) {
     if (P.kind != CFDG_PRECOND_SGS) return;
     if (P.order == CFDG_SGS_SEQUENTIAL) build_seq_groups(c);
@@ -285,6 +290,10 @@ void precond_setup(cfdg_ctx& c, const Precond& P) {
     cfdg::launch_lu_factor(c.nc, c.diag.get(), c.values.get(), c.storage, c.lu.get(), c.lu_f32, c.piv.get(),
                            c.singular.get(), c.stream);
     c.check_launch();
+}
+
+void precond_check(cfdg_ctx& c, const Precond& P) {
+    if (P.kind != CFDG_PRECOND_SGS) return;
     int sing = 0;
     c.d2h(&sing, c.singular.get(), 1);
     c.sync();
@@ -299,61 +308,39 @@ void precond_setup(cfdg_ctx& c, const Precond& P) {
 // exit.  Scalars and dot products are FP64.
 template <class T>
 cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& P, const KVecs<T>& w, T* x,
-                          const T* bvec, bool zero_fallback = false) {
+                          const T* bvec, bool zero_fallback = false, bool x_zero = false) {
     const long long N = 5LL * c.nc;
     const std::size_t bytes = static_cast<std::size_t>(N) * sizeof(T);
     cfdg_iter_result res{CFDG_MAX_ITERATIONS, 0, 0.0, 0.0};
-
-    spmv_stored<T>(c, x, w.v);
-    cfdg::launch_axpy_copy<T>(N, bvec, -1.0, w.v, w.r, c.stream);
+    double* ks = c.ks.get();
+    double* h = c.h_ks;
+    // r = b - A x (r = b for a zero guess), norms, tolerance and status on the
+    // device: no host round trip before the first batch of iterations
     {
         cfdg::DotPairsT<T> pr{};
         pr.a[0] = bvec;
         pr.b[0] = bvec;
-        pr.a[1] = w.r;
-        pr.b[1] = w.r;
-        cfdg::launch_dots(N, 2, pr, c.partial.get(), c.dots.get(), c.stream);
-        fetch_dots(c, 2);
-    }
-    const double bnorm = std::sqrt(c.h_dots[0]);
-    double rnorm = std::sqrt(c.h_dots[1]);
-    if (zero_fallback) {
-        // learned initial guess: keep it only if it lowers the initial residual
-        if (!(rnorm <= bnorm)) {
-            hip_check(hipMemsetAsync(x, 0, bytes, c.stream), "memset");
+        if (x_zero) {
             hip_check(hipMemcpyAsync(w.r, bvec, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
-            rnorm = bnorm;
-            ++c.warm_rejected;
         } else {
-            ++c.warm_used;
+            spmv_stored<T>(c, x, w.v);
+            cfdg::launch_axpy_copy<T>(N, bvec, -1.0, w.v, w.r, c.stream);
+            pr.a[1] = w.r;
+            pr.b[1] = w.r;
         }
-    }
-    const double scale = (bnorm > 0.0) ? bnorm : 1.0;
-    const double tol = std::max(lp.abs_tol, lp.rel_tol * scale);
-    res.initial_residual = rnorm;
-    if (!std::isfinite(rnorm)) {
-        res.status = CFDG_DIVERGED;
-    } else if (rnorm <= tol) {
-        res.status = CFDG_CONVERGED;
+        cfdg::launch_dots(N, x_zero ? 1 : 2, pr, c.partial.get(), c.dots.get(), c.stream);
+        cfdg::launch_bicg_init(ks, c.dots.get(), x_zero ? 1 : 0, zero_fallback ? 1 : 0, lp.abs_tol, lp.rel_tol,
+                               lp.max_iter, c.stream);
+        if (zero_fallback) cfdg::launch_bicg_reject<T>(N, ks, bvec, x, w.r, c.stream);
     }
     int it = 0;
-    if (res.status == CFDG_MAX_ITERATIONS) {
+    double rnorm = 0.0;
+    {
         hip_check(hipMemcpyAsync(w.r0, w.r, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
         hip_check(hipMemsetAsync(w.v, 0, bytes, c.stream), "memset");
         // The recurrence runs on the device: scalars in the state block ks,
         // iterations enqueued in batches and the state read back once per
         // batch; iterations past convergence or breakdown are no-ops.
-        double* ks = c.ks.get();
-        double* h = c.h_ks;
-        for (int k = 0; k < cfdg::kKsCount; ++k) h[k] = 0.0;
-        h[cfdg::kKsMaxIt] = static_cast<double>(lp.max_iter);
-        h[cfdg::kKsTol] = tol;
-        h[cfdg::kKsRho] = rnorm * rnorm;
-        h[cfdg::kKsRhoOld] = 1.0;
-        h[cfdg::kKsAlpha] = 1.0;
-        h[cfdg::kKsOmega] = 1.0;
-        h[cfdg::kKsRnorm] = rnorm;
-        c.h2d(ks, h, cfdg::kKsCount);
         const int slot = (sizeof(T) == sizeof(double)) ? 0 : 1;
         int batch = std::max(1, c.last_iters[slot] - 1);
         long long enq = 0;
@@ -408,7 +395,14 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
         const double st = h[cfdg::kKsStatus];
         res.status = (st == cfdg::kKsConverged) ? CFDG_CONVERGED
                                                  : (st == cfdg::kKsDiverged ? CFDG_DIVERGED : CFDG_MAX_ITERATIONS);
-        c.last_iters[slot] = it;
+        res.initial_residual = h[cfdg::kKsRnorm0];
+        if (zero_fallback) {
+            if (h[cfdg::kKsReject] != 0.0)
+                ++c.warm_rejected;
+            else
+                ++c.warm_used;
+        }
+        if (it > 0) c.last_iters[slot] = it;
     }
     c.check_launch();
     c.sync();
@@ -421,7 +415,7 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
 // x (FP64, initial guess on entry), b (FP64).  Every pass forms r = b - A x in
 // FP64 arithmetic over the stored blocks and solves A e = r in FP32 from e = 0.
 cfdg_iter_result defect_correction(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& P, double* x,
-                                   const double* bvec, bool zero_fallback = false) {
+                                   const double* bvec, bool zero_fallback = false, bool x_zero = false) {
     const long long N = 5LL * c.nc;
     cfdg_iter_result res{CFDG_MAX_ITERATIONS, 0, 0.0, 0.0};
     const auto outer_residual = [&]() {
@@ -429,12 +423,32 @@ cfdg_iter_result defect_correction(cfdg_ctx& c, const cfdg_linparams& lp, const 
         cfdg::launch_axpy_copy<double>(N, bvec, -1.0, c.v.get(), c.r.get(), c.stream);
         return std::sqrt(dot_device<double>(c, c.r.get(), c.r.get()));
     };
-    const double bnorm = std::sqrt(dot_device<double>(c, bvec, bvec));
-    double rnorm = outer_residual();
+    // ||b|| and the first outer residual in one reduction (r = b for a zero guess)
+    double bnorm = 0.0, rnorm = 0.0;
+    {
+        cfdg::DotPairsT<double> pr{};
+        pr.a[0] = bvec;
+        pr.b[0] = bvec;
+        if (x_zero) {
+            hip_check(hipMemcpyAsync(c.r.get(), bvec, static_cast<std::size_t>(N) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c.stream), "copy");
+        } else {
+            spmv_stored<double>(c, x, c.v.get());
+            cfdg::launch_axpy_copy<double>(N, bvec, -1.0, c.v.get(), c.r.get(), c.stream);
+            pr.a[1] = c.r.get();
+            pr.b[1] = c.r.get();
+        }
+        cfdg::launch_dots(N, x_zero ? 1 : 2, pr, c.partial.get(), c.dots.get(), c.stream);
+        fetch_dots(c, x_zero ? 1 : 2);
+        bnorm = std::sqrt(c.h_dots[0]);
+        rnorm = x_zero ? bnorm : std::sqrt(c.h_dots[1]);
+    }
     if (zero_fallback) {
         if (!(rnorm <= bnorm)) {
             c.x.zero(c.stream);
-            rnorm = outer_residual();
+            hip_check(hipMemcpyAsync(c.r.get(), bvec, static_cast<std::size_t>(N) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c.stream), "copy");
+            rnorm = bnorm;
             ++c.warm_rejected;
         } else {
             ++c.warm_used;
@@ -467,7 +481,7 @@ cfdg_iter_result defect_correction(cfdg_ctx& c, const cfdg_linparams& lp, const 
         in.rel_tol = c.prec.inner_rel_tol;
         in.abs_tol = 0.9 * tol;
         in.max_iter = lp.max_iter - total;
-        const cfdg_iter_result ir = bicgstab<float>(c, in, P, w, c.f32.x.get(), c.f32.b.get());
+        const cfdg_iter_result ir = bicgstab<float>(c, in, P, w, c.f32.x.get(), c.f32.b.get(), false, true);
         total += ir.iterations;
         if (ir.iterations == 0 && ir.status != CFDG_CONVERGED) {
             res.status = CFDG_DIVERGED;
@@ -499,7 +513,7 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
         if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "implicit right-hand side needs a residual");
         cfdg::launch_rhs(c.nc, c.nc, c.res.get(), c.b.get(), c.stream);
     }
-    bool learned = false;
+    bool learned = false, x_zero = false;
     if (hx0) {
         const auto d = c.vec_to_device(hx0);
         c.h2d(c.x.get(), d.data(), n5);
@@ -508,6 +522,7 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
         learned = cfdg::gnn_warm_guess(c, c.x.get());
     } else {
         c.x.zero(c.stream);
+        x_zero = true;
     }
     Precond P;
     P.kind = lp.preconditioner;
@@ -515,10 +530,11 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
     hip_check(hipEventRecord(c.ev[5], c.stream), "event");
     precond_setup(c, P);
     const cfdg_iter_result r = (c.prec.krylov == CFDG_PREC_FP32)
-                                   ? defect_correction(c, lp, P, c.x.get(), c.b.get(), learned)
-                                   : bicgstab<double>(c, lp, P, kvecs64(c), c.x.get(), c.b.get(), learned);
+                                   ? defect_correction(c, lp, P, c.x.get(), c.b.get(), learned, x_zero)
+                                   : bicgstab<double>(c, lp, P, kvecs64(c), c.x.get(), c.b.get(), learned, x_zero);
     hip_check(hipEventRecord(c.ev[6], c.stream), "event");
     c.sync();
+    precond_check(c, P);
     if (hx) {
         std::vector<double> d(n5);
         c.d2h(d.data(), c.x.get(), n5);
@@ -1083,6 +1099,7 @@ int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* zout
         c->d2h(h.data(), c->sh.get(), n5);
         c->check_launch();
         c->sync();
+        precond_check(*c, P);
         c->vec_to_mesh(h, zout);
     });
 }

@@ -180,7 +180,10 @@ __host__ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // --- residual stages (kernels_residual.hip) ---------------------------------
 void launch_primitives(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
 void launch_ghosts(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs, hipStream_t s);
-void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
+// With sa given and the LDS-staged kernel in use, the module's nu_tilde gradient
+// is formed in the same pass; returns whether it was.
+bool launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s,
+                             const SaArgs* sa = nullptr);
 // inverse LSQ normal matrices (planes kRInv..), once the centroids and the vertex stencil are on the device
 void launch_lsq_setup(const MeshArgs& m, const FieldArgs& f, hipStream_t s);
 void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BcTable& bcs,

@@ -3,8 +3,9 @@
 //   k_lu_factor   diagonal-block LU with partial pivoting (bsr_helpers.hpp:278-321)
 //   k_spmv        y = A x, five lanes per block row (bsr_helpers.hpp:139-203)
 //   k_sgs_group   one Gauss-Seidel row update for every row of one row group
-//                 (bsr_helpers.hpp:390-435 for a single row); a sweep launches
-//                 the groups in order (colours, or dependency levels)
+//                 (bsr_helpers.hpp:390-435 for a single row), eight lanes per
+//                 row; a sweep launches the groups in order (colours, or
+//                 dependency levels)
 //   vector ops    the BiCGSTAB updates of bicgstab.cpp:97-177, fused
 //   reductions    dot products: per-block partial sums in a fixed order, then
 //                 one block adds the partials in a fixed order -> the result
@@ -117,41 +118,51 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, cons
 }
 
 // One Gauss-Seidel update for each row of a group: z_i = LU_i^-1 (r_i - sum_{k != d} A_ik z_k).
+// Eight lanes per row (five active): lane ii forms component ii of the sum over
+// the row's off-diagonal blocks, reading block row ii (5 contiguous values), so
+// the five lanes of a row read each 5x5 block as one contiguous 25-value run;
+// the five sums are then exchanged within the group of eight and every lane
+// performs the same 5x5 LU solve (lu_solve_block<5>, bsr_helpers.hpp:325-353)
+// and keeps its component.  The arithmetic and its order per component are
+// those of one thread per row.
 template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
                                                       const int* diag, const TS* values, const TL* lu,
                                                       const std::uint8_t* piv, const TV* r, TV* z, const double* gate) {
     if (gate && gate[kKsStatus] != 0.0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int i = rows[t];
-    const int d = diag[i];
-    TV s[5];
-    for (int ii = 0; ii < 5; ++ii) s[ii] = r[static_cast<long long>(i) * 5 + ii];
-    const int kb = row_ptr[i], ke = row_ptr[i + 1];
-    for (int k = kb; k < ke; ++k) {
-        if (k == d) continue;
-        const TS* a = values + static_cast<long long>(k) * 25;
-        const TV* zb = z + static_cast<long long>(cols[k]) * 5;
-        const TV z0 = zb[0], z1 = zb[1], z2 = zb[2], z3 = zb[3], z4 = zb[4];
-        for (int ii = 0; ii < 5; ++ii) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long slot = t >> 3;
+    const int ii = static_cast<int>(t & 7);
+    const bool has_row = slot < count;  // uniform over the group of eight
+    const int i = has_row ? rows[slot] : 0;
+    TV acc = 0;
+    if (has_row && ii < 5) {
+        acc = r[static_cast<long long>(i) * 5 + ii];
+        const int d = diag[i];
+        const int kb = row_ptr[i], ke = row_ptr[i + 1];
+        for (int k = kb; k < ke; ++k) {
+            if (k == d) continue;
+            const TS* a = values + static_cast<long long>(k) * 25 + ii * 5;
+            const TV* zb = z + static_cast<long long>(cols[k]) * 5;
             TV tt = 0;
-            tt += load_as<TV>(a[ii * 5 + 0]) * z0;
-            tt += load_as<TV>(a[ii * 5 + 1]) * z1;
-            tt += load_as<TV>(a[ii * 5 + 2]) * z2;
-            tt += load_as<TV>(a[ii * 5 + 3]) * z3;
-            tt += load_as<TV>(a[ii * 5 + 4]) * z4;
-            s[ii] -= tt;
+            tt += load_as<TV>(a[0]) * zb[0];
+            tt += load_as<TV>(a[1]) * zb[1];
+            tt += load_as<TV>(a[2]) * zb[2];
+            tt += load_as<TV>(a[3]) * zb[3];
+            tt += load_as<TV>(a[4]) * zb[4];
+            acc -= tt;
         }
     }
-    // lu_solve_block<5> (bsr_helpers.hpp:325-353)
+    const int base = static_cast<int>(threadIdx.x & 63) & ~7;
+    TV s[5];
+    for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, base + q, 64);
+    if (!has_row || ii >= 5) return;
     const TL* L = lu + static_cast<long long>(i) * 25;
     const std::uint8_t* p = piv + static_cast<long long>(i) * 5;
     for (int k = 0; k < 5; ++k) {
         const int pk = p[k];
         if (pk != k) {
             TV tmp = s[k];
-            // s[pk] with a compile-time-friendly select
             TV sp = s[0];
             for (int q = 1; q < 5; ++q) sp = (q == pk) ? s[q] : sp;
             s[k] = sp;
@@ -159,16 +170,18 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
         }
     }
     for (int a = 1; a < 5; ++a) {
-        TV acc = s[a];
-        for (int b = 0; b < a; ++b) acc -= load_as<TV>(L[a * 5 + b]) * s[b];
-        s[a] = acc;
+        TV acc2 = s[a];
+        for (int b = 0; b < a; ++b) acc2 -= load_as<TV>(L[a * 5 + b]) * s[b];
+        s[a] = acc2;
     }
     for (int a = 4; a >= 0; --a) {
-        TV acc = s[a];
-        for (int b = a + 1; b < 5; ++b) acc -= load_as<TV>(L[a * 5 + b]) * s[b];
-        s[a] = acc / load_as<TV>(L[a * 5 + a]);
+        TV acc2 = s[a];
+        for (int b = a + 1; b < 5; ++b) acc2 -= load_as<TV>(L[a * 5 + b]) * s[b];
+        s[a] = acc2 / load_as<TV>(L[a * 5 + a]);
     }
-    for (int ii = 0; ii < 5; ++ii) z[static_cast<long long>(i) * 5 + ii] = s[ii];
+    TV out = s[0];
+    for (int q = 1; q < 5; ++q) out = (q == ii) ? s[q] : out;
+    z[static_cast<long long>(i) * 5 + ii] = out;
 }
 
 // p = ((p * beta) + (-beta * omega) v) + r           (bicgstab.cpp:104-109)
@@ -213,6 +226,44 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(long long n, double alpha,
 // iteration returns at once when ks[kKsStatus] is no longer "running", so the
 // iterations enqueued after convergence or breakdown change nothing and the
 // result is the one the per-iteration host loop of bicgstab.cpp:42-201 gives.
+
+// Start of a solve (bicgstab.cpp:56-92): norms, tolerance, initial status.
+__global__ void k_bicg_init(double* ks, const double* dots, int x_zero, int fallback, double abs_tol,
+                            double rel_tol, int max_iter) {
+    const double bnorm = sqrt(dots[0]);
+    double rnorm = x_zero ? bnorm : sqrt(dots[1]);
+    double reject = 0.0;
+    if (fallback && !(rnorm <= bnorm)) {
+        reject = 1.0;
+        rnorm = bnorm;
+    }
+    const double scale = (bnorm > 0.0) ? bnorm : 1.0;
+    const double tol = dmax(abs_tol, rel_tol * scale);
+    for (int k = 0; k < kKsCount; ++k) ks[k] = 0.0;
+    ks[kKsMaxIt] = static_cast<double>(max_iter);
+    ks[kKsTol] = tol;
+    ks[kKsRho] = rnorm * rnorm;
+    ks[kKsRhoOld] = 1.0;
+    ks[kKsAlpha] = 1.0;
+    ks[kKsOmega] = 1.0;
+    ks[kKsRnorm] = rnorm;
+    ks[kKsRnorm0] = rnorm;
+    ks[kKsReject] = reject;
+    if (!isfinite(rnorm))
+        ks[kKsStatus] = kKsDiverged;
+    else if (rnorm <= tol)
+        ks[kKsStatus] = kKsConverged;
+}
+
+// A rejected guess restarts from zero: x = 0, r = b.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_bicg_reject(long long n, const double* ks, const T* b, T* x, T* r) {
+    if (ks[kKsReject] == 0.0) return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = static_cast<T>(0);
+    r[i] = b[i];
+}
 
 // Start of an iteration (bicgstab.cpp:97-109): iteration limit, it += 1,
 // beta = (rho / rho_old)(alpha / omega) from the second iteration on.
@@ -478,7 +529,7 @@ template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
                       const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate) {
     if (count <= 0) return;
-    const dim3 g = grid_for(count), blk(kBlock);
+    const dim3 g = grid_for(8LL * count), blk(kBlock);
     if constexpr (sizeof(TV) == sizeof(double)) {
         if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
         hipLaunchKernelGGL((k_sgs_group<double, double, double>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
@@ -556,6 +607,18 @@ template void launch_dots<double>(long long, int, const DotPairsT<double>&, doub
                                   const double*);
 template void launch_dots<float>(long long, int, const DotPairsT<float>&, double*, double*, hipStream_t,
                                  const double*);
+
+void launch_bicg_init(double* ks, const double* dots, int x_zero, int fallback, double abs_tol, double rel_tol,
+                      int max_iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1), 0, s, ks, dots, x_zero, fallback, abs_tol, rel_tol, max_iter);
+}
+
+template <class T>
+void launch_bicg_reject(long long n, const double* ks, const T* b, T* x, T* r, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_bicg_reject<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, b, x, r);
+}
+template void launch_bicg_reject<double>(long long, const double*, const double*, double*, double*, hipStream_t);
+template void launch_bicg_reject<float>(long long, const double*, const float*, float*, float*, hipStream_t);
 
 void launch_bicg_begin(double* ks, hipStream_t s) { hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(1), 0, s, ks); }
 void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s) {

@@ -20,7 +20,7 @@ constexpr int kReduceBlocks = 1024;  // partial sums per reduction (fixed -> det
 // once when it is not 0.
 enum : int {
     kKsStatus = 0, kKsIt, kKsMaxIt, kKsTol, kKsRho, kKsRhoOld, kKsAlpha, kKsOmega, kKsBeta, kKsNbo, kKsFirst,
-    kKsOmegaNew, kKsRnNew, kKsRhoNew, kKsRnorm, kKsCount = 16
+    kKsOmegaNew, kKsRnNew, kKsRhoNew, kKsRnorm, kKsRnorm0, kKsReject, kKsCount = 24
 };
 constexpr double kKsConverged = 1.0, kKsDiverged = 2.0, kKsMaxIter = 3.0;
 
@@ -59,6 +59,14 @@ template <class T>
 void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s,
                  const double* gate = nullptr);
 // device-driven BiCGSTAB: one-thread scalar steps and gated vector updates
+// Start of a solve from dots = (b, b) and, unless x_zero, (r, r) with r = b - A x:
+// tolerance max(abs_tol, rel_tol ||b||), status, rho; with fallback, a guess
+// whose residual exceeds ||b|| is marked rejected (launch_bicg_reject then
+// restarts from zero).
+void launch_bicg_init(double* ks, const double* dots, int x_zero, int fallback, double abs_tol, double rel_tol,
+                      int max_iter, hipStream_t s);
+template <class T>
+void launch_bicg_reject(long long n, const double* ks, const T* b, T* x, T* r, hipStream_t s);
 void launch_bicg_begin(double* ks, hipStream_t s);
 void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s);
 void launch_bicg_omega(double* ks, const double* dots, hipStream_t s);

@@ -185,9 +185,33 @@ __global__ void __launch_bounds__(kBlock) k_lsq_setup(MeshArgs m, FieldArgs f) {
 // laid out as the first half of a cell record (q, centroid) for the cell, its
 // j-th vertex neighbour and its j-th face neighbour -- either the records in
 // planes in HBM or their copies staged in LDS.
-template <class Self, class NbN, class NbF>
+// Optional nu_tilde gradient formed in the same least-squares pass (the
+// turbulence module's scalar, same coefficients, same order per term).
+struct NoScalar {
+    static constexpr bool kOn = false;
+    __device__ double self() const { return 0.0; }
+    __device__ double nb(int) const { return 0.0; }
+    __device__ void store(int, const double*) const {}
+};
+struct LdsScalar {
+    static constexpr bool kOn = true;
+    const double* s;       // LDS slots
+    const std::uint16_t* slot;  // slot of each vertex-neighbour entry
+    int own;               // slot of the cell
+    double* g;             // output: 3 planes of nc
+    long long nc;
+    __device__ double self() const { return s[own]; }
+    __device__ double nb(int j) const { return s[slot[j]]; }
+    __device__ void store(int c, const double* v) const {
+        g[c] = v[0];
+        g[nc + c] = v[1];
+        g[2 * nc + c] = v[2];
+    }
+};
+
+template <class Self, class NbN, class NbF, class Sc = NoScalar>
 __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
-                                                      const Self& self, NbN nbn, NbF nbf) {
+                                                      const Self& self, NbN nbn, NbF nbf, const Sc& sc = Sc{}) {
     const CellRefW rc = crecw(f, c);
     double qc[5];
     for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
@@ -200,6 +224,8 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
         // inverse normal matrix of the cell (set-up time, lsq_inverse): the
         // coefficients C_j = A^-1 w_j dr_j are applied on the fly
         const int jb = m.ccn_off[c], je = m.ccn_off[c + 1];
+        double gs[3] = {0.0, 0.0, 0.0};
+        const double sc0 = sc.self();
         if (je > jb) {
             const CellRef ri{f.cs, f.ld, c};
             const double i0 = ri[kRInv], i1 = ri[kRInv + 1], i2 = ri[kRInv + 2], i4 = ri[kRInv + 3],
@@ -219,8 +245,15 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
                     g[v][1] += cy * d;
                     g[v][2] += cz * d;
                 }
+                if constexpr (Sc::kOn) {
+                    const double d = sc.nb(j) - sc0;
+                    gs[0] += cx * d;
+                    gs[1] += cy * d;
+                    gs[2] += cz * d;
+                }
             }
         }
+        if constexpr (Sc::kOn) sc.store(c, gs);
     } else {
         // Green-Gauss: the face scatter gathered per cell (both the face- and
         // cell-based variants form these sums)
@@ -303,8 +336,9 @@ struct LdsSlot {
 // its stencil halo are staged once into LDS (structure of arrays, so that
 // lanes reading different slots of one plane spread over the banks), then
 // every cell reads its vertex and face neighbours from there.
-__global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm) {
-    __shared__ double sq[8 * kGradSlots];
+template <bool kSA>
+__global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
+    __shared__ double sq[(kSA ? 9 : 8) * kGradSlots];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
@@ -315,12 +349,19 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
             sq[k * kGradSlots + sl] = v.x;
             sq[(k + 1) * kGradSlots + sl] = v.y;
         }
+        if constexpr (kSA) sq[8 * kGradSlots + sl] = sa.nu[src];
     }
     __syncthreads();
     if (static_cast<int>(threadIdx.x) >= nt) return;
-    gradient_limiter_cell(
-        m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)},
-        [&](int j) { return LdsSlot{sq, m.ccn_slot[j]}; }, [&](int j) { return LdsSlot{sq, m.ccf_slot[j]}; });
+    const auto nbn = [&](int j) { return LdsSlot{sq, m.ccn_slot[j]}; };
+    const auto nbf = [&](int j) { return LdsSlot{sq, m.ccf_slot[j]}; };
+    if constexpr (kSA) {
+        // the module's nu_tilde gradient in the same pass (k_sa_gradient's arithmetic)
+        const LdsScalar sc{sq + 8 * kGradSlots, m.ccn_slot, static_cast<int>(threadIdx.x), sa.g, m.nc};
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, sc);
+    } else {
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf);
+    }
 }
 
 __device__ __forceinline__ void grad_dirichlet(double* g, const double* gi, double qin, double qfix, double nx,
@@ -477,12 +518,25 @@ __global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, Nu
     __shared__ double sF[6][kTileFaces];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
+    // the next face's index and cells are loaded one iteration ahead, so the
+    // cell-record loads of a face issue as soon as its iteration starts
+    int nfc = 0, ncx = 0, ncy = 0;
+    if (static_cast<int>(threadIdx.x) < nfl) {
+        const int j = fb + threadIdx.x;
+        nfc = __builtin_nontemporal_load(m.tile_faces + j);
+        ncx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j));
+        ncy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j) + 1);
+    }
     for (int i = threadIdx.x; i < nfl; i += kTileCells) {
         double F[5], lam_f;
-        const int fc = __builtin_nontemporal_load(m.tile_faces + fb + i);
-        const int cx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + fb + i));
-        const int cy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + fb + i) + 1);
-        const int2 cc = make_int2(cx, cy);
+        const int fc = nfc;
+        const int2 cc = make_int2(ncx, ncy);
+        if (i + kTileCells < nfl) {
+            const int j = fb + i + kTileCells;
+            nfc = __builtin_nontemporal_load(m.tile_faces + j);
+            ncx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j));
+            ncy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j) + 1);
+        }
         face_flux<kTurb>(f, nm, FaceRefNT{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
         if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
         for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
@@ -635,13 +689,21 @@ void launch_ghosts(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, con
     if (nbf > 0) hipLaunchKernelGGL(k_ghosts, grid_for(nbf), dim3(kBlock), 0, s, m, f, nm, bcs);
 }
 
-void launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s) {
-    if (m.nc <= 0) return;
-    if (m.tile_halo_off != nullptr)
-        hipLaunchKernelGGL(k_gradient_limiter_lds, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f,
-                           nm);
-    else
-        hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
+bool launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s,
+                             const SaArgs* sa) {
+    if (m.nc <= 0) return false;
+    if (m.tile_halo_off != nullptr) {
+        if (sa) {
+            hipLaunchKernelGGL(k_gradient_limiter_lds<true>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0,
+                               s, m, f, nm, *sa);
+            return true;
+        }
+        hipLaunchKernelGGL(k_gradient_limiter_lds<false>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s,
+                           m, f, nm, SaArgs{});
+        return false;
+    }
+    hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
+    return false;
 }
 
 void launch_lsq_setup(const MeshArgs& m, const FieldArgs& f, hipStream_t s) {

@@ -208,3 +208,71 @@ def test_boundary_integrals_match_reference_golden(name):
     scale = np.abs(fx["ref_res"]).reshape(5, -1).sum(axis=1)
     assert np.all(np.abs(mg - fx["ref_bnd_mass"]) <= 1e-12 * scale[0] + 1e-300)
     assert np.all(np.abs(eg - fx["ref_bnd_energy"]) <= 1e-12 * scale[4] + 1e-300)
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 5])
+def test_linear_solve_iteration_limit_matches_oracle(max_iter):
+    """The device-driven BiCGSTAB stops where the reference's loop stops
+    (bicgstab.cpp:97-201): the iteration limit (status MAX_ITERATIONS after
+    exactly max_iter iterations), the same iterate (1e-9) and residual."""
+    case = CASES["plate"]
+    mesh, gpu, ora = _pair(case)
+    for s in (gpu, ora):
+        s.eval_residual()
+        s.compute_dt()
+        s.assemble_jacobian()
+    lp = cfdg.cases.default_linparams(rel_tol=1e-14, abs_tol=0.0, max_iter=max_iter, order=cfdg.SGS_SEQUENTIAL)
+    xg, rg = gpu.linear_solve(lp)
+    xo, ro = ora.linear_solve(lp)
+    assert rg.status == ro.status == cfdg.MAX_ITERATIONS
+    assert rg.iterations == ro.iterations == max_iter
+    if max_iter:
+        assert_close_scaled(xg, xo, np.abs(xo).max(), 1e-9, "iterate at the limit")
+    else:
+        assert not np.any(xg)
+    assert abs(rg.final_residual - ro.final_residual) <= 1e-9 * ro.initial_residual
+
+
+def test_linear_solve_zero_rhs_and_tolerance_edges():
+    """b = 0 converges before the first iteration (tolerance max(abs, rel ||b||)
+    with ||b|| = 0 -> the scale 1 of bicgstab.cpp:69-72); a loose tolerance met
+    by the initial residual also stops with zero iterations."""
+    case = CASES["plate"]
+    mesh, gpu, ora = _pair(case)
+    for s in (gpu, ora):
+        s.eval_residual()
+        s.compute_dt()
+        s.assemble_jacobian()
+    n = 5 * gpu.n_cells
+    lp = cfdg.cases.default_linparams(order=cfdg.SGS_SEQUENTIAL)
+    xg, rg = gpu.linear_solve(lp, b=np.zeros(n))
+    xo, ro = ora.linear_solve(lp, b=np.zeros(n))
+    assert rg.status == ro.status == cfdg.CONVERGED and rg.iterations == ro.iterations == 0
+    assert not np.any(xg)
+    loose = cfdg.cases.default_linparams(rel_tol=2.0, order=cfdg.SGS_SEQUENTIAL)
+    xg, rg = gpu.linear_solve(loose)
+    xo, ro = ora.linear_solve(loose)
+    assert rg.status == ro.status == cfdg.CONVERGED and rg.iterations == ro.iterations == 0
+
+
+def test_singular_diagonal_block_is_reported():
+    """A zero diagonal block makes the SGS set-up fail (preconditioners.cpp:84-89
+    is fatal in the reference): the solve reports CFDG_ERR_SINGULAR naming the
+    mesh row, for the solve and for a single preconditioner application."""
+    case = CASES["plate"]
+    mesh, gpu, _ = _pair(case)
+    gpu.eval_residual()
+    gpu.compute_dt()
+    gpu.assemble_jacobian()
+    rp, cols, dg = gpu.bsr_pattern()
+    vals = gpu.bsr_values()
+    row = 7
+    vals[dg[row]] = 0.0
+    gpu.set_bsr_values(vals)
+    lp = cfdg.cases.default_linparams(order=cfdg.SGS_SEQUENTIAL)
+    with pytest.raises(cfdg.CfdgError) as e:
+        gpu.linear_solve(lp, b=np.ones(5 * gpu.n_cells))
+    assert e.value.code == cfdg.ERR_SINGULAR and f"row {row}" in str(e.value)
+    with pytest.raises(cfdg.CfdgError) as e:
+        gpu.sgs_apply(cfdg.SGS_SEQUENTIAL, np.ones(5 * gpu.n_cells))
+    assert e.value.code == cfdg.ERR_SINGULAR

@@ -1,8 +1,9 @@
 """Spalart-Allmaras module on the GPU against the CPU oracle.
 
-Case: flat plate 24x16x8 at Re_L = 5e6 (viscous + SA, node LSQ, MUSCL),
-freestream with 2 % seeded noise in U and 20 % in rho nu_tilde; a second run
-uses Green-Gauss gradients so every gradient path of the module is covered.
+Cases: flat plate 24x16x8 at Re_L = 5e6 (viscous + SA, MUSCL) with each of
+the three gradient methods, the supersonic ramp with no-slip walls at
+Re_L = 5e6 (inflow / outflow zones) and the tetrahedral bump; freestream
+with 2 % seeded noise in U and 20 % in rho nu_tilde.
 
 Tolerances (FP64 on both sides, same summation order):
   * eddy viscosity, nu_tilde, its gradient, density: 1e-12 relative;
@@ -48,9 +49,25 @@ def _pair(case, update=cfdg.TURB_POINT_IMPLICIT, seed=3):
     return mesh, s, o
 
 
+SA_GEOMETRIES = {
+    # supersonic ramp: inflow and outflow zones, the freestream override of inflowing faces
+    "ramp_sa": lambda: cases.ramp_sa(),
+    # Kuhn tetrahedra (gather gradient kernel, 70-neighbour stencils)
+    "bump_sa": lambda: cases.bump_sa(ni=8, nj=8, nk=8),
+}
+
+
 @pytest.mark.parametrize("gradient", ["LEAST_SQUARES_NODE", "GREEN_GAUSS_FACE", "GREEN_GAUSS_CELL"])
 def test_sa_residual_matches_oracle(gradient):
-    case = _case(gradient)
+    _check_residual(_case(gradient))
+
+
+@pytest.mark.parametrize("name", sorted(SA_GEOMETRIES))
+def test_sa_residual_matches_oracle_other_geometries(name):
+    _check_residual(SA_GEOMETRIES[name]())
+
+
+def _check_residual(case):
     mesh, s, o = _pair(case)
     s.eval_residual()
     o.eval_residual()
