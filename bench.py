@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--drop-size", default="64x32x16",
                     help="plate mesh of the time-to-1e-8-drop runs ('off' skips them)")
     ap.add_argument("--drop-max-steps", type=int, default=12000)
+    ap.add_argument("--sa", choices=["on", "off"], default="on",
+                    help="also time the residual with the Spalart-Allmaras module on the same mesh")
     ap.add_argument("--pmc-run", action="store_true",
                     help="profiling mode: one 1 GiB device copy (counter calibration) and flux-kernel launches only")
     return ap.parse_args()
@@ -195,10 +197,49 @@ def main():
         "implicit_step_forms": imp_forms,
         "setup_s": {"mesh": round(t_mesh, 2), "context": round(t_setup, 2)},
     }
+    solver.close()
+    if a.sa != "off":
+        out["sa_residual"] = sa_residual(a)
     if a.drop_size != "off":
-        solver.close()
         out["time_to_drop"] = time_to_drop(a)
     print(json.dumps(out))
+
+
+def sa_residual(a):
+    """Residual evaluation with the Spalart-Allmaras module on the plate mesh
+    at Re_L = 5e6: the mean-flow residual with eddy viscosity plus the
+    module's ghosts, gradient, face sweep and sources, device-resident."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    ni, nj, nk = (int(x) for x in a.size.split("x"))
+    case = cases.plate_sa(ni=ni, nj=nj, nk=nk)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s.init_uniform(case.rho, case.velocity, case.p)
+    U = s.get_state()
+    s.set_state(U * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal(U.shape)))
+    t0 = time.perf_counter()
+    s.set_turbulence(cases.turbulence(case))
+    t_set = time.perf_counter() - t0
+    for _ in range(a.warmup):
+        s.eval_residual()
+    t = []
+    for _ in range(max(a.steps // 5, 5)):
+        t1 = time.perf_counter()
+        s.eval_residual()
+        t.append(time.perf_counter() - t1)
+    ms = 1e3 * float(np.median(t))
+    stages = s.stage_times()
+    nc = mesh.n_cells
+    s.close()
+    return {"workload": f"flat plate {ni}x{nj}x{nk} ({nc} cells), Re_L 5e6, viscous + Spalart-Allmaras "
+                        "(point-implicit module), node LSQ + Venkatakrishnan MUSCL",
+            "ms": round(ms, 4), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
+            "stage_ms": [round(float(x), 4) for x in stages[:4]],
+            "set_turbulence_s": round(t_set, 2)}
 
 
 def time_to_drop(a):
