@@ -37,8 +37,10 @@ def parse():
     ap.add_argument("--implicit-steps", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "off"], default="auto")
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--drop-size", default="64x32x16",
+    ap.add_argument("--drop-size", default="128x64x32",
                     help="plate mesh of the time-to-1e-8-drop runs ('off' skips them)")
+    ap.add_argument("--drop-previous-update", action="store_true",
+                    help="also time the previous-update guess alone (the network with all parameters zero)")
     ap.add_argument("--drop-max-steps", type=int, default=12000)
     ap.add_argument("--sa", choices=["on", "off"], default="on",
                     help="also time the residual with the Spalart-Allmaras module on the same mesh")
@@ -57,8 +59,8 @@ def flux_algorithmic_bytes(nc, nbf, nf, viscous=True):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (requested --gpus {a.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print(f"bench.py: one GPU only (requested --gpus {a.gpus}); the path is not sharded across GPUs",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -251,9 +253,9 @@ def time_to_drop(a):
     CFL 1000, from the freestream with seeded 2 % noise.  The network is
     trained first on 8 seeded 32x16x8 plates with Mach drawn from U(0.2, 0.5)
     (every 100th of 2000 steps recorded, so the samples span the whole
-    convergence history); its training time is reported separately.  The
-    previous-update guess alone (the network with all parameters zero) is
-    timed too, so the network's own share of the learned form shows."""
+    convergence history); its training time is reported separately.  With
+    --drop-previous-update the previous-update guess alone (the network with
+    all parameters zero) is timed too, so the network's own share shows."""
     import numpy as np
 
     import cfdg
@@ -292,9 +294,10 @@ def time_to_drop(a):
                        "CFL 1000, BiCGSTAB + multicolour SGS (rel 0.1), 2% seeded noise on the freestream"}
     out["plain_fp64"], U0 = run("plain")
     out["fp32_krylov"], U1 = run("fp32")
-    zero = cfdg.Gnn(seed=1)
-    zero.set_params(np.zeros(zero.num_params, np.float32))
-    out["fp32_krylov_previous_update"], _ = run("learned", zero)
+    if a.drop_previous_update:
+        zero = cfdg.Gnn(seed=1)
+        zero.set_params(np.zeros(zero.num_params, np.float32))
+        out["fp32_krylov_previous_update"], _ = run("learned", zero)
     t0 = time.perf_counter()
     gnn = cfdg.Gnn(seed=11)
     rng = np.random.default_rng(2024)

@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU restatement of the reference solver's single-rank implicit
  * iteration.  TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the
- * CPU-baseline leg of the benchmark link it as the checker; the product never
+ * CPU-baseline leg of the benchmark link it to compare against; the product never
  * does.
  *
  * The structs below have exactly the layout of their cfdg.h counterparts so a

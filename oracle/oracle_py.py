@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py as the checker.  The API mirrors cfdg.Solver so
+CPU-baseline leg of bench.py, to compare against.  The API mirrors cfdg.Solver so
 a test runs the same calls on both and compares.  Argument blocks are the
 cfdg ctypes structures (identical layouts, see oracle.h).
 """
