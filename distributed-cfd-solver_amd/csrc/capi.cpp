@@ -677,10 +677,21 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         const cfdg::Csr st = to_device_rows(cfdg::cell_cells_by_node(m), c.new2old, &c.old2new);
         c.ccn_off.upload(st.off, s);
         c.ccn.upload(st.idx, s);
-        // LDS-staged stencils when every tile's cells + halo fit the kernel's slots
+        // LDS-staged stencils: the flux tiles when their cells + halo fit the
+        // kernel's slots, else consecutive sub-ranges of them that do
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
-        const cfdg::TileStencil ts = cfdg::tile_stencil(tl, st, ccf_dev);
+        std::vector<i32> goff = tl.tile_off;
+        cfdg::TileStencil ts = cfdg::tile_stencil(goff, st, ccf_dev);
+        if (ts.max_slots > cfdg::kGradSlots) {
+            goff = cfdg::split_tiles(tl.tile_off, st, ccf_dev, cfdg::kGradSlots);
+            ts = cfdg::tile_stencil(goff, st, ccf_dev);
+        }
         if (ts.max_slots <= cfdg::kGradSlots) {
+            int most = 0;
+            for (std::size_t k = 0; k + 1 < goff.size(); ++k) most = std::max(most, goff[k + 1] - goff[k]);
+            c.ngtiles = static_cast<int>(goff.size()) - 1;
+            c.gblock = std::min(cfdg::kTileCells, (most + 63) / 64 * 64);
+            c.gtile_off.upload(goff, s);
             c.tile_halo_off.upload(ts.halo_off, s);
             c.tile_halo.upload(ts.halo, s);
             c.ccn_slot.upload(ts.ccn_slot, s);

@@ -88,7 +88,8 @@ struct cfdg_ctx {
     DBuf<int2> fcells, tcells;    // (owner, neighbour or ghost) per face / per tile face-list entry
     long long fld = 0, tld = 0;
     DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
-    DBuf<int> tile_halo_off, tile_halo;
+    DBuf<int> tile_halo_off, tile_halo, gtile_off;  // gradient tiles (the flux tiles or sub-ranges of them)
+    int ngtiles = 0, gblock = 0;
     DBuf<std::uint16_t> ccn_slot, ccf_slot;
     // fields (device order)
     DBuf<double> U, cs, res, lam, dt, alpha, bflux, bint;
@@ -188,6 +189,9 @@ struct cfdg_ctx {
         a.tile_face_off = tile_face_off.get();
         a.tile_faces = tile_faces.get();
         a.tile_halo_off = tile_halo_off.get();
+        a.gtile_off = gtile_off.get();
+        a.ngtiles = ngtiles;
+        a.gblock = gblock;
         a.tile_halo = tile_halo.get();
         a.ccn_slot = ccn_slot.get();
         a.ccf_slot = ccf_slot.get();

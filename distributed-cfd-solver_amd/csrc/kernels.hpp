@@ -111,9 +111,12 @@ struct MeshArgs {
     const int* tile_off;      // tiles + 1
     const int* tile_face_off; // tiles + 1
     const int* tile_faces;
-    // LDS-staged gradient stencils (null when the tiles' stencils do not fit)
-    const int* tile_halo_off; // tiles + 1
-    const int* tile_halo;     // halo cells of each tile, ascending device id
+    // LDS-staged gradient stencils over the gradient tiles (null when they do not fit)
+    const int* gtile_off;     // gradient tiles + 1: the flux tiles, or consecutive sub-ranges of them
+    int ngtiles;
+    int gblock;               // threads per gradient workgroup (multiple of 64, >= cells of any gradient tile)
+    const int* tile_halo_off; // gradient tiles + 1
+    const int* tile_halo;     // halo cells of each gradient tile, ascending device id
     const std::uint16_t* ccn_slot;  // LDS slot of each ccn entry
     const std::uint16_t* ccf_slot;  // LDS slot of each ccf entry
 };

@@ -332,17 +332,19 @@ struct LdsSlot {
     __device__ __forceinline__ double operator[](int k) const { return sq[k * kGradSlots + s]; }
 };
 
-// One workgroup per tile: the (q, centroid) planes of the tile's cells and of
-// its stencil halo are staged once into LDS (structure of arrays, so that
-// lanes reading different slots of one plane spread over the banks), then
-// every cell reads its vertex and face neighbours from there.
+// One workgroup per gradient tile (a flux tile, or a consecutive sub-range of
+// one where the tile's stencil halo would not fit; blockDim.x >= its cells):
+// the (q, centroid) planes of the tile's cells and of its stencil halo are
+// staged once into LDS (structure of arrays, so that lanes reading different
+// slots of one plane spread over the banks), then every cell reads its vertex
+// and face neighbours from there.
 template <bool kSA>
 __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
     __shared__ double sq[(kSA ? 9 : 8) * kGradSlots];
     const int t = xcd_block(blockIdx.x, gridDim.x);
-    const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
+    const int c0 = m.gtile_off[t], nt = m.gtile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
-    for (int sl = threadIdx.x; sl < ns; sl += kTileCells) {
+    for (int sl = threadIdx.x; sl < ns; sl += blockDim.x) {
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
         for (int k = 0; k < 8; k += 2) {  // planes 0-7 = q, centroid: four 16-byte pair loads
             const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(k, f.ld, src));
@@ -693,13 +695,12 @@ bool launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
                              const SaArgs* sa) {
     if (m.nc <= 0) return false;
     if (m.tile_halo_off != nullptr) {
+        const dim3 g(static_cast<unsigned>(m.ngtiles)), b(static_cast<unsigned>(m.gblock));
         if (sa) {
-            hipLaunchKernelGGL(k_gradient_limiter_lds<true>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0,
-                               s, m, f, nm, *sa);
+            hipLaunchKernelGGL(k_gradient_limiter_lds<true>, g, b, 0, s, m, f, nm, *sa);
             return true;
         }
-        hipLaunchKernelGGL(k_gradient_limiter_lds<false>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s,
-                           m, f, nm, SaArgs{});
+        hipLaunchKernelGGL(k_gradient_limiter_lds<false>, g, b, 0, s, m, f, nm, SaArgs{});
         return false;
     }
     hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);

@@ -357,16 +357,17 @@ TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>&
     return tf;
 }
 
-TileStencil tile_stencil(const Tiling& t, const Csr& dev_ccn, const Csr& dev_ccf) {
+TileStencil tile_stencil(const std::vector<i32>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf) {
     TileStencil ts;
-    const std::size_t ntiles = t.tile_off.size() - 1;
-    const std::size_t n = t.new2old.size();
+    const std::size_t ntiles = tile_off.size() - 1;
+    const std::size_t n = dev_ccn.off.size() - 1;
+    const std::vector<i32>& toff = tile_off;
     ts.halo_off.assign(ntiles + 1, 0);
     ts.ccn_slot.assign(dev_ccn.idx.size(), 0);
     ts.ccf_slot.assign(dev_ccf.idx.size(), 0);
     std::vector<i32> slot(n, -1), local;
     for (std::size_t k = 0; k < ntiles; ++k) {
-        const i32 c0 = t.tile_off[k], c1 = t.tile_off[k + 1];
+        const i32 c0 = toff[k], c1 = toff[k + 1];
         local.clear();
         for (i32 c = c0; c < c1; ++c) {
             for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
@@ -397,6 +398,54 @@ TileStencil tile_stencil(const Tiling& t, const Csr& dev_ccn, const Csr& dev_ccf
         ts.halo_off[k + 1] = static_cast<i32>(ts.halo.size());
     }
     return ts;
+}
+
+std::vector<i32> split_tiles(const std::vector<i32>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf, int max_slots) {
+    const std::size_t n = dev_ccn.off.size() - 1;
+    std::vector<i32> out{0};
+    std::vector<i32> mark(n, -1);  // chunk id that counts the cell as halo
+    i32 chunk = 0;
+    const auto for_nb = [&](i32 c, auto&& fn) {
+        for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) fn(dev_ccn.idx[z(e)]);
+        for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) fn(dev_ccf.idx[z(e)]);
+    };
+    for (std::size_t k = 0; k + 1 < tile_off.size(); ++k) {
+        const i32 t0 = tile_off[k], t1 = tile_off[k + 1];
+        i32 start = t0, halo = 0;
+        ++chunk;
+        for (i32 c = t0; c < t1; ++c) {
+            // slots if c joins [start, c]: own cells + halo (c leaves the halo if it was in it)
+            i32 fresh = 0;
+            for_nb(c, [&](i32 nb) {
+                if ((nb < start || nb > c) && mark[z(nb)] != chunk) {
+                    mark[z(nb)] = -2 - chunk;  // counted once in this trial
+                    ++fresh;
+                }
+            });
+            for_nb(c, [&](i32 nb) {
+                if (mark[z(nb)] == -2 - chunk) mark[z(nb)] = -1;
+            });
+            const i32 was_halo = (mark[z(c)] == chunk) ? 1 : 0;
+            if (c > start && (c + 1 - start) + (halo - was_halo + fresh) > max_slots) {
+                out.push_back(c);  // close the chunk before c
+                start = c;
+                halo = 0;
+                ++chunk;
+            }
+            if (mark[z(c)] == chunk) {
+                --halo;
+                mark[z(c)] = -1;
+            }
+            for_nb(c, [&](i32 nb) {
+                if ((nb < start || nb > c) && mark[z(nb)] != chunk) {
+                    mark[z(nb)] = chunk;
+                    ++halo;
+                }
+            });
+        }
+        out.push_back(t1);
+    }
+    return out;
 }
 
 std::vector<i32> backward_levels(const Csr& ccf, i32 n, i32& n_levels) {

@@ -98,6 +98,12 @@ struct TileStencil {
     std::vector<std::uint16_t> ccf_slot;
     std::int32_t max_slots = 0;
 };
-TileStencil tile_stencil(const Tiling& t, const Csr& dev_ccn, const Csr& dev_ccf);
+TileStencil tile_stencil(const std::vector<std::int32_t>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf);
+
+// Splits each tile (a range of device cells) into consecutive sub-ranges whose
+// cells plus stencil halo take at most max_slots LDS slots, for meshes whose
+// full tiles do not fit (tetrahedra: about 70 vertex neighbours per cell).
+std::vector<std::int32_t> split_tiles(const std::vector<std::int32_t>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf,
+                                      int max_slots);
 
 }  // namespace cfdg
