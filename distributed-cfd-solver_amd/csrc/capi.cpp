@@ -609,7 +609,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     c.ntiles = static_cast<int>(tl.tile_off.size()) - 1;
 
     // face planes (mesh face order) and (owner, neighbour or ghost) in device numbering
-    std::vector<i32> owner_dev(z(c.nf));
+    std::vector<i32> owner_dev(z(c.nf)), other_dev(z(c.nf));
     std::vector<double> fph;
     std::vector<int2> fcells(z(c.nf));
     c.fld = (static_cast<long long>(c.nf) + 63) / 64 * 64;
@@ -632,6 +632,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
             put(cfdg::kFDinv, f, di[z(f)]);
             owner_dev[z(f)] = c.old2new[z(m.face_owner[z(f)])];
             const i32 other = (f < c.ninner) ? c.old2new[z(m.face_neigh[z(f)])] : c.nc + (f - c.ninner);
+            other_dev[z(f)] = other;
             fcells[z(f)] = int2{owner_dev[z(f)], other};
         }
         c.fp.upload(fph, s);
@@ -640,7 +641,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     }
     {
         const cfdg::Csr cf_dev = to_device_rows(cf_mesh, c.new2old, nullptr);
-        const cfdg::TileFaces tf = cfdg::tile_faces(tl, cf_dev, owner_dev);
+        const cfdg::TileFaces tf = cfdg::tile_faces(tl, cf_dev, owner_dev, other_dev);
         for (int k = 0; k < c.ntiles; ++k) {
             if (tf.tile_face_off[z(k) + 1] - tf.tile_face_off[z(k)] > cfdg::kTileFaces ||
                 tl.tile_off[z(k) + 1] - tl.tile_off[z(k)] > cfdg::kTileCells)
@@ -652,19 +653,20 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.tile_off.upload(tl.tile_off, s);
         c.tile_face_off.upload(tf.tile_face_off, s);
         c.tile_faces.upload(tf.faces, s);
-        // the flux kernel's copy of the face planes, in the order of the tiles' face lists
+        // the flux kernel's copy of the face planes, in the order of the tiles'
+        // face lists: normal, area, centroid, and the packed (owner, other) ids
         const long long nte = static_cast<long long>(tf.faces.size());
         c.tld = (nte + 63) / 64 * 64;
-        std::vector<double> tph(z(c.tld) * cfdg::kFacePlanes, 0.0);
-        std::vector<int2> tcells(z(nte));
+        std::vector<double> tph(z(c.tld) * cfdg::kTilePlanes, 0.0);
         for (long long e = 0; e < nte; ++e) {
             const i32 f = tf.faces[z(e)];
-            for (int k = 0; k < cfdg::kFacePlanes; ++k)
+            for (int k = 0; k < cfdg::kTCells; ++k)
                 tph[z(cfdg::plane_at(k, c.tld, e))] = fph[z(cfdg::plane_at(k, c.fld, f))];
-            tcells[z(e)] = fcells[z(f)];
+            const std::uint64_t ids = static_cast<std::uint32_t>(fcells[z(f)].x) |
+                                      (static_cast<std::uint64_t>(static_cast<std::uint32_t>(fcells[z(f)].y)) << 32);
+            std::memcpy(&tph[z(cfdg::plane_at(cfdg::kTCells, c.tld, e))], &ids, sizeof ids);
         }
         c.tf.upload(tph, s);
-        c.tcells.upload(tcells, s);
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
         c.ccf_off.upload(ccf_dev.off, s);
         c.ccf.upload(ccf_dev.idx, s);

@@ -85,7 +85,7 @@ struct cfdg_ctx {
 
     // mesh (device)
     DBuf<double> fp, tf;          // face planes: mesh order (fld), tile face-list order (tld)
-    DBuf<int2> fcells, tcells;    // (owner, neighbour or ghost) per face / per tile face-list entry
+    DBuf<int2> fcells;            // (owner, neighbour or ghost) per face
     long long fld = 0, tld = 0;
     DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
     DBuf<int> tile_halo_off, tile_halo, gtile_off;  // gradient tiles (the flux tiles or sub-ranges of them)
@@ -176,7 +176,6 @@ struct cfdg_ctx {
         a.fcells = fcells.get();
         a.tf = tf.get();
         a.tld = tld;
-        a.tcells = tcells.get();
         a.cf_off = cf_off.get();
         a.cf = cf.get();
         a.cf_slot = cf_slot.get();

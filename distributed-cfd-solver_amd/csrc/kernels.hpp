@@ -19,6 +19,7 @@ namespace cfdg {
 constexpr int kMaxPatches = 64;
 constexpr int kTileCells = 256;       // cells per tile (= threads per flux workgroup)
 constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux kernel
+constexpr int kFluxThreads = 512;     // threads per flux workgroup (kTileFaces / kFluxThreads faces per lane)
 constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kGradSlots = 640;       // LDS slots (tile cells + stencil halo) of the staged gradient kernel
 // storage of the Jacobian blocks (values of the precision enumeration of cfdg.h)
@@ -43,10 +44,15 @@ constexpr int kRInv = 30;  // inverse LSQ normal matrix (00, 01, 02, 11, 12, 22)
 // (unit normal, area, centroid, owner->neighbour centroid vector -- mirrored
 // 2 (x_f - x_c) at boundary faces -- and its inverse length), plus
 // (owner, neighbour) device ids per face, the neighbour of a boundary face
-// being its ghost n_cells + (f - n_inner).  The flux kernel reads a second
-// copy of the same planes laid out in the order of the tiles' face lists.
+// being its ghost n_cells + (f - n_inner).  The flux kernel reads a second,
+// shorter copy laid out in the order of the tiles' face lists: the first
+// kTilePlanes - 1 planes (normal, area, centroid) and, in plane kTCells, the
+// face's (owner, neighbour or ghost) ids packed into the 64 bits of the
+// double (owner in the low half); it forms the centroid vector itself.
 constexpr int kFacePlanes = 12;  // 11 used, paired like the cell planes
 constexpr int kFN = 0, kFA = 3, kFC = 4, kFD = 7, kFDinv = 10;
+constexpr int kTilePlanes = 8, kTCells = 7;
+static_assert(kTCells % 2 == 1 && kTCells == kFC + 3, "packed ids share a pair with the centroid z");
 
 // Position of quantity k of entry i in paired planes of stride ld.
 __host__ __device__ __forceinline__ long long plane_at(int k, long long ld, long long i) {
@@ -97,9 +103,8 @@ struct MeshArgs {
     const double* fp;         // kFacePlanes x fld, mesh face order
     long long fld;
     const int2* fcells;       // (owner, neighbour or ghost) per face, device ids
-    const double* tf;         // kFacePlanes x tld, tile face-list order
+    const double* tf;         // kTilePlanes x tld, tile face-list order
     long long tld;
-    const int2* tcells;       // (owner, neighbour or ghost) per tile face-list entry
     const int* cf_off;        // device cell -> faces, ascending face index
     const int* cf;
     const int* cf_slot;       // position of the face in the tile's face list (negative: neighbour side)

@@ -458,11 +458,14 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
 // contribution (residual_kernel.hpp:183-240 / 245-300 for one face, with the
 // MUSCL states of muscl.hpp:108-157).
 // fr views the face's planes, (c0, c1) = (owner, neighbour or ghost).
-template <bool kTurb, class FR>
-__device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm, const FR& fr, bool inner, int c0,
-                                          int c1, double F[5], double& lam_f) {
-    const CellRef r0 = crec(f, c0);
-    const CellRef r1 = crec(f, c1);
+// r0 / r1 view the owner's and the other cell's planes (in HBM or staged in
+// LDS), mutL / mutR are their eddy viscosities (zero without turbulence).
+// The owner -> neighbour centroid vector and its inverse length are formed
+// here with the operations of face_distances (aux_geometry.cpp:11-51,120-156),
+// so they equal the mesh-order face planes bit for bit.
+template <class FR, class R0, class R1>
+__device__ __forceinline__ void face_flux(const NumArgs& nm, const FR& fr, bool inner, const R0& r0, const R1& r1,
+                                          double mutL, double mutR, double F[5], double& lam_f) {
     const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
     double qL[5], qR[5];
     if (inner && nm.muscl) {
@@ -497,69 +500,160 @@ __device__ __forceinline__ void face_flux(const FieldArgs& f, const NumArgs& nm,
             ga[k] = r0[kRG + 3 + k];
             gb[k] = r1[kRG + 3 + k];
         }
+        double dvx, dvy, dvz, inv_d;
+        if (inner) {
+            const double ex = r1[kRC] - r0[kRC], ey = r1[kRC + 1] - r0[kRC + 1], ez = r1[kRC + 2] - r0[kRC + 2];
+            dvx = ex;
+            dvy = ey;
+            dvz = ez;
+            inv_d = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+        } else {
+            const double ex = fr[kFC] - r0[kRC], ey = fr[kFC + 1] - r0[kRC + 1], ez = fr[kFC + 2] - r0[kRC + 2];
+            dvx = 2.0 * ex;
+            dvy = 2.0 * ey;
+            dvz = 2.0 * ez;
+            inv_d = 1.0 / sqrt(4.0 * ex * ex + 4.0 * ey * ey + 4.0 * ez * ez);
+        }
         double Fv[5];
         double lv = 0.0;
-        // eddy viscosity of both sides (residual_kernel.hpp:229-230, 282-283)
-        const double mutL = kTurb ? f.mut[c0] : 0.0;
-        const double mutR = kTurb ? f.mut[c1] : 0.0;
-        viscous_flux(nm.gas, fr[kFDinv], fr[kFD], fr[kFD + 1], fr[kFD + 2], qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A,
-                     Fv, lv);
+        viscous_flux(nm.gas, inv_d, dvx, dvy, dvz, qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A, Fv, lv);
         for (int v = 1; v < 5; ++v) F[v] += Fv[v];
         lam_f += lv;
     }
 }
 
-// Tiled flux residual: each face touching the tile is evaluated once (face-
-// parallel) into LDS, then each cell adds its faces' fluxes in ascending face
-// index, with + for the owner and - for the neighbour (residual_kernel.hpp:
-// 175-306 formed as a gather; only faces on a tile boundary are evaluated by
-// both adjacent tiles).  With turbulence the kernel also records each face's
-// mass flux F[0] (from the tile that owns the face's owner cell).
-template <bool kTurb>
-__global__ void __launch_bounds__(kTileCells) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
-    __shared__ double sF[6][kTileFaces];
-    const int t = xcd_block(blockIdx.x, gridDim.x);
-    const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
-    // the next face's index and cells are loaded one iteration ahead, so the
-    // cell-record loads of a face issue as soon as its iteration starts
-    int nfc = 0, ncx = 0, ncy = 0;
-    if (static_cast<int>(threadIdx.x) < nfl) {
-        const int j = fb + threadIdx.x;
-        nfc = __builtin_nontemporal_load(m.tile_faces + j);
-        ncx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j));
-        ncy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j) + 1);
+// View of a tile cell's planes staged in LDS: planes 0 .. kFluxPlanes-1 in
+// the pair layout of the cell planes with a stride of kTileCells slots.
+constexpr int kFluxPlanes = 30;  // q, centroid, limiter, volume, gradients (plane 29 unused)
+struct LdsRec {
+    const double* s;
+    int i;  // slot = device id - tile start
+    __device__ __forceinline__ double operator[](int k) const { return s[((k >> 1) * kTileCells + i) * 2 + (k & 1)]; }
+};
+
+// One entry of the tile face-list stream (non-temporal loads); the centroid's
+// z, which shares its 16-byte pair with the packed cell ids, is held by value.
+struct TileFaceRef {
+    const double* b;
+    long long ld;
+    int e;
+    double fz;
+    __device__ __forceinline__ double operator[](int k) const {
+        if (k == kFC + 2) return fz;
+        const double* base = b + static_cast<long long>(k >> 1) * 2 * ld;
+        return __builtin_nontemporal_load(base + (2u * static_cast<unsigned>(e) + static_cast<unsigned>(k & 1)));
     }
-    for (int i = threadIdx.x; i < nfl; i += kTileCells) {
-        double F[5], lam_f;
-        const int fc = nfc;
-        const int2 cc = make_int2(ncx, ncy);
-        if (i + kTileCells < nfl) {
-            const int j = fb + i + kTileCells;
-            nfc = __builtin_nontemporal_load(m.tile_faces + j);
-            ncx = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j));
-            ncy = __builtin_nontemporal_load(reinterpret_cast<const int*>(m.tcells + j) + 1);
+};
+
+// Tiled flux residual: each face touching the tile is evaluated once (face-
+// parallel, one face per lane) and each cell then adds its faces' fluxes in
+// ascending face index, with + for the owner and - for the neighbour
+// (residual_kernel.hpp:175-306 formed as a gather; only faces on a tile
+// boundary are evaluated by both adjacent tiles).  The tile's own cells are
+// staged into LDS first (coalesced), so only the halo and ghost side of a
+// face on the tile boundary is read from HBM; the face list is grouped by
+// which cells lie in the tile, so a wave mostly takes one of the three load
+// paths.  After the face phase the staging area is reused for the fluxes.
+// With turbulence the kernel also records each face's mass flux F[0] (from
+// the tile that owns the face's owner cell); eddy viscosity follows the
+// cells (residual_kernel.hpp:229-230, 282-283).
+template <bool kTurb>
+__global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) k_flux(MeshArgs m, FieldArgs f, NumArgs nm) {
+    __shared__ double sc[kFluxPlanes * kTileCells];  // staged planes, then the fluxes sF[6][kTileFaces]
+    __shared__ double smut[kTurb ? kTileCells : 1];
+    __shared__ int scf[kTileCells + 1 + 2 * kTileFaces];  // the tile's cf_off (rebased), then its cf_slot entries
+    static_assert(6 * kTileFaces <= kFluxPlanes * kTileCells, "flux area must fit the staging area");
+    constexpr int kPerLane = kTileFaces / kFluxThreads;  // faces per lane
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
+    const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
+    const int i = threadIdx.x;
+    int fc[kPerLane], cx[kPerLane], cy[kPerLane];
+    double fz[kPerLane];
+#pragma unroll
+    for (int r = 0; r < kPerLane; ++r) {  // this lane's faces: cells (and index), issued before the staging loads
+        const int j = i + r * kFluxThreads;
+        fc[r] = cx[r] = cy[r] = 0;
+        fz[r] = 0.0;
+        if (j < nfl) {
+            typedef double pair_t __attribute__((ext_vector_type(2)));
+            const pair_t w = __builtin_nontemporal_load(
+                reinterpret_cast<const pair_t*>(m.tf + plane_at(kTCells - 1, m.tld, fb + j)));
+            const unsigned long long ids = static_cast<unsigned long long>(__double_as_longlong(w.y));
+            fz[r] = w.x;
+            cx[r] = static_cast<int>(static_cast<unsigned>(ids));
+            cy[r] = static_cast<int>(static_cast<unsigned>(ids >> 32));
+            if (kTurb) fc[r] = __builtin_nontemporal_load(m.tile_faces + fb + j);
         }
-        face_flux<kTurb>(f, nm, FaceRefNT{m.tf, m.tld, fb + i}, fc < m.ninner, cc.x, cc.y, F, lam_f);
-        if (kTurb && cc.x >= m.tile_off[t] && cc.x < m.tile_off[t + 1]) f.mdot[fc] = F[0];
-        for (int v = 0; v < 5; ++v) sF[v][i] = F[v];
-        sF[5][i] = lam_f;
-        if (fc >= m.ninner)  // a boundary face lies in exactly one tile: single writer
-            for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(fc - m.ninner) * 5 + v] = F[v];
+    }
+    // staging of the plane pairs the face fluxes read: q (pairs 0-2), centroid
+    // and limiter (2-6) for MUSCL, gradients (7-14) for MUSCL or viscous terms
+    {
+        const int sl = i % kTileCells, g = i / kTileCells;
+        constexpr int kGroups = kFluxThreads / kTileCells, kPairs = kFluxPlanes / 2;
+        const int p_hi = (nm.muscl || nm.viscous) ? kPairs : 3;
+        if (sl < nt) {
+            for (int p = g; p < p_hi; p += kGroups) {
+                if (p > 2 && p < 7 && !nm.muscl) continue;
+                const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(2 * p, f.ld, c0 + sl));
+                *reinterpret_cast<double2*>(sc + (p * kTileCells + sl) * 2) = v;
+            }
+            if (kTurb && g == 0) smut[sl] = f.mut[c0 + sl];
+        }
+        // the cells' face slots for the gather: a contiguous range of cf_slot
+        // (at most two entries per tile face); ne > 2 kTileFaces cannot occur
+        const int e0 = m.cf_off[c0], ne = m.cf_off[c0 + nt] - e0;
+        if (i <= nt) scf[i] = m.cf_off[c0 + i] - e0;
+        for (int e = i; e < ne; e += kFluxThreads) scf[kTileCells + 1 + e] = m.cf_slot[e0 + e];
     }
     __syncthreads();
-    const int c = m.tile_off[t] + threadIdx.x;
-    if (c >= m.tile_off[t + 1]) return;
+    double F[kPerLane][6];
+#pragma unroll
+    for (int r = 0; r < kPerLane; ++r) {
+        const int j = i + r * kFluxThreads;
+        for (int v = 0; v < 6; ++v) F[r][v] = 0.0;
+        if (j >= nfl) break;  // later faces of the lane are out of range too
+        const TileFaceRef fr{m.tf, m.tld, fb + j, fz[r]};
+        const bool inner = cy[r] < m.nc;  // boundary faces: the other cell is a ghost
+        const unsigned s0 = static_cast<unsigned>(cx[r] - c0), s1 = static_cast<unsigned>(cy[r] - c0);
+        const bool in0 = s0 < static_cast<unsigned>(nt), in1 = s1 < static_cast<unsigned>(nt);
+        double* Fr = F[r];
+        if (in0 && in1) {
+            face_flux(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, LdsRec{sc, static_cast<int>(s1)},
+                      kTurb ? smut[s0] : 0.0, kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
+        } else if (in0) {
+            face_flux(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, crec(f, cy[r]), kTurb ? smut[s0] : 0.0,
+                      kTurb ? f.mut[cy[r]] : 0.0, Fr, Fr[5]);
+        } else {
+            face_flux(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
+                      kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
+        }
+        if (kTurb && in0) f.mdot[fc[r]] = Fr[0];
+        if (!inner)  // a boundary face lies in exactly one tile: single writer; ghost id = cells + boundary index
+            for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(cy[r] - m.nc) * 5 + v] = Fr[v];
+    }
+    __syncthreads();  // every read of the staged planes is done
+    double* sF = sc;
+#pragma unroll
+    for (int r = 0; r < kPerLane; ++r) {
+        const int j = i + r * kFluxThreads;
+        if (j < nfl)
+            for (int v = 0; v < 6; ++v) sF[v * kTileFaces + j] = F[r][v];
+    }
+    __syncthreads();
+    if (i >= nt) return;
+    const int c = c0 + i;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double lam = 0.0;
-    for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const int s = m.cf_slot[e];
+    for (int e = scf[i]; e < scf[i + 1]; ++e) {
+        const int s = scf[kTileCells + 1 + e];
         if (s >= 0) {
-            for (int v = 0; v < 5; ++v) acc[v] += sF[v][s];
-            lam += sF[5][s];
+            for (int v = 0; v < 5; ++v) acc[v] += sF[v * kTileFaces + s];
+            lam += sF[5 * kTileFaces + s];
         } else {
             const int k = -s - 1;
-            for (int v = 0; v < 5; ++v) acc[v] -= sF[v][k];
-            lam += sF[5][k];
+            for (int v = 0; v < 5; ++v) acc[v] -= sF[v * kTileFaces + k];
+            lam += sF[5 * kTileFaces + k];
         }
     }
     for (int v = 0; v < 5; ++v) __builtin_nontemporal_store(acc[v], f.res + static_cast<long long>(v) * m.nc + c);
@@ -720,9 +814,9 @@ void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s) {
     if (m.ntiles <= 0) return;
     if (f.mut)
-        hipLaunchKernelGGL(k_flux<true>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
+        hipLaunchKernelGGL(k_flux<true>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kFluxThreads), 0, s, m, f, nm);
     else
-        hipLaunchKernelGGL(k_flux<false>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kTileCells), 0, s, m, f, nm);
+        hipLaunchKernelGGL(k_flux<false>, dim3(static_cast<unsigned>(m.ntiles)), dim3(kFluxThreads), 0, s, m, f, nm);
 }
 
 void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patches, double* out, hipStream_t s) {

@@ -332,26 +332,38 @@ Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, i
     return t;
 }
 
-TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>& face_owner_dev) {
+TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>& face_owner_dev,
+                     const std::vector<i32>& face_other_dev) {
     TileFaces tf;
     const std::size_t ntiles = t.tile_off.size() - 1;
     tf.tile_face_off.assign(ntiles + 1, 0);
     tf.cell_slot.assign(dev_cf.idx.size(), 0);
-    std::vector<i32> local;
+    std::vector<i32> pos(face_owner_dev.size(), -1);
+    std::vector<std::pair<int, i32>> local;  // (group, face)
     for (std::size_t k = 0; k < ntiles; ++k) {
+        const i32 c0 = t.tile_off[k], c1 = t.tile_off[k + 1];
+        const auto inside = [&](i32 c) { return c >= c0 && c < c1; };
         local.clear();
-        for (i32 c = t.tile_off[k]; c < t.tile_off[k + 1]; ++c)
-            for (i32 e = dev_cf.off[z(c)]; e < dev_cf.off[z(c) + 1]; ++e) local.push_back(dev_cf.idx[z(e)]);
-        std::sort(local.begin(), local.end());
-        local.erase(std::unique(local.begin(), local.end()), local.end());
-        for (i32 c = t.tile_off[k]; c < t.tile_off[k + 1]; ++c) {
+        for (i32 c = c0; c < c1; ++c)
             for (i32 e = dev_cf.off[z(c)]; e < dev_cf.off[z(c) + 1]; ++e) {
                 const i32 f = dev_cf.idx[z(e)];
-                const i32 slot = static_cast<i32>(std::lower_bound(local.begin(), local.end(), f) - local.begin());
+                const bool in0 = inside(face_owner_dev[z(f)]), in1 = inside(face_other_dev[z(f)]);
+                local.emplace_back(in0 ? (in1 ? 0 : 1) : 2, f);
+            }
+        std::sort(local.begin(), local.end());
+        local.erase(std::unique(local.begin(), local.end()), local.end());
+        for (std::size_t j = 0; j < local.size(); ++j) {
+            pos[z(local[j].second)] = static_cast<i32>(j);
+            tf.faces.push_back(local[j].second);
+        }
+        for (i32 c = c0; c < c1; ++c) {
+            for (i32 e = dev_cf.off[z(c)]; e < dev_cf.off[z(c) + 1]; ++e) {
+                const i32 f = dev_cf.idx[z(e)];
+                const i32 slot = pos[z(f)];
                 tf.cell_slot[z(e)] = (face_owner_dev[z(f)] == c) ? slot : -slot - 1;
             }
         }
-        tf.faces.insert(tf.faces.end(), local.begin(), local.end());
+        for (std::size_t j = 0; j < local.size(); ++j) pos[z(local[j].second)] = -1;
         tf.tile_face_off[k + 1] = static_cast<i32>(tf.faces.size());
     }
     return tf;

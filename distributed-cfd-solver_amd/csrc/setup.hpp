@@ -75,15 +75,21 @@ Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, i
 
 // Per-tile face lists for the tiled flux kernel, in device numbering:
 //   faces[tile_face_off[t] .. tile_face_off[t+1]) = the faces touching tile t,
-//   ascending face index; cell_slot[e] (parallel to the device cell->faces
-//   list) = position of that face in its tile's list, negated and minus one
-//   when the cell is the face's neighbour (the flux enters with a minus sign).
+//   grouped by which of the face's cells lie in the tile -- both, the owner
+//   only (boundary faces included: their ghost is outside every tile), the
+//   neighbour only -- and ascending face index within each group, so that the
+//   lanes of a wave mostly take the same path; cell_slot[e] (parallel to the
+//   device cell->faces list) = position of that face in its tile's list,
+//   negated and minus one when the cell is the face's neighbour (the flux
+//   enters with a minus sign).  face_other_dev = neighbour device id, or the
+//   ghost id (>= cells) of a boundary face.
 struct TileFaces {
     std::vector<std::int32_t> tile_face_off;
     std::vector<std::int32_t> faces;
     std::vector<std::int32_t> cell_slot;
 };
-TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<std::int32_t>& face_owner_dev);
+TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<std::int32_t>& face_owner_dev,
+                     const std::vector<std::int32_t>& face_other_dev);
 
 // LDS-staged neighbour stencils for the gradient kernel, in device numbering.
 // A tile's LDS slots are its own cells (slot = cell - tile start) followed by
