@@ -698,6 +698,35 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
             c.tile_halo.upload(ts.halo, s);
             c.ccn_slot.upload(ts.ccn_slot, s);
             c.ccf_slot.upload(ts.ccf_slot, s);
+            c.ccn_base.upload(ts.ccn_base, s);
+            c.ccf_base.upload(ts.ccf_base, s);
+            // limiter face vectors x_f - x_c of every cell face (ascending face
+            // index), per gradient tile transposed like the slot lists: tile t
+            // holds 3 components x W_t faces x nt cells, W_t = its longest list
+            const cfdg::Csr cf_dev = to_device_rows(cf_mesh, c.new2old, nullptr);
+            std::vector<long long> fvb(goff.size(), 0);
+            for (std::size_t k = 0; k + 1 < goff.size(); ++k) {
+                i32 w = 0;
+                for (i32 cc = goff[k]; cc < goff[k + 1]; ++cc) w = std::max(w, cf_dev.off[z(cc) + 1] - cf_dev.off[z(cc)]);
+                fvb[k + 1] = fvb[k] + 3LL * w * (goff[k + 1] - goff[k]);
+            }
+            std::vector<double> fv(z(fvb.back()), 0.0);
+            for (std::size_t k = 0; k + 1 < goff.size(); ++k) {
+                const i32 nt = goff[k + 1] - goff[k];
+                const long long w = (fvb[k + 1] - fvb[k]) / (3LL * nt);
+                for (i32 cc = goff[k]; cc < goff[k + 1]; ++cc) {
+                    const std::size_t mc = z(c.new2old[z(cc)]);
+                    for (i32 e = cf_dev.off[z(cc)]; e < cf_dev.off[z(cc) + 1]; ++e) {
+                        const std::size_t f = z(cf_dev.idx[z(e)]);
+                        const long long at = fvb[k] + (e - cf_dev.off[z(cc)]) * static_cast<long long>(nt) + (cc - goff[k]);
+                        fv[z(at)] = m.face_cx[f] - m.cell_cx[mc];
+                        fv[z(at + w * nt)] = m.face_cy[f] - m.cell_cy[mc];
+                        fv[z(at + 2 * w * nt)] = m.face_cz[f] - m.cell_cz[mc];
+                    }
+                }
+            }
+            c.face_vec.upload(fv, s);
+            c.face_vec_base.upload(fvb, s);
         }
         c.sync();
     }

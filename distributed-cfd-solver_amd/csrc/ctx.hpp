@@ -91,6 +91,9 @@ struct cfdg_ctx {
     DBuf<int> tile_halo_off, tile_halo, gtile_off;  // gradient tiles (the flux tiles or sub-ranges of them)
     int ngtiles = 0, gblock = 0;
     DBuf<std::uint16_t> ccn_slot, ccf_slot;
+    DBuf<int> ccn_base, ccf_base;
+    DBuf<double> face_vec;        // limiter face vectors per gradient tile (transposed)
+    DBuf<long long> face_vec_base;
     // fields (device order)
     DBuf<double> U, cs, res, lam, dt, alpha, bflux, bint;
     long long ld = 0;  // cell plane stride
@@ -194,6 +197,10 @@ struct cfdg_ctx {
         a.tile_halo = tile_halo.get();
         a.ccn_slot = ccn_slot.get();
         a.ccf_slot = ccf_slot.get();
+        a.ccn_base = ccn_base.get();
+        a.ccf_base = ccf_base.get();
+        a.face_vec = face_vec.get();
+        a.face_vec_base = face_vec_base.get();
         return a;
     }
     cfdg::FieldArgs fargs() const {

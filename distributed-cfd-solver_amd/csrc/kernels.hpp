@@ -122,8 +122,18 @@ struct MeshArgs {
     int gblock;               // threads per gradient workgroup (multiple of 64, >= cells of any gradient tile)
     const int* tile_halo_off; // gradient tiles + 1
     const int* tile_halo;     // halo cells of each gradient tile, ascending device id
-    const std::uint16_t* ccn_slot;  // LDS slot of each ccn entry
-    const std::uint16_t* ccf_slot;  // LDS slot of each ccf entry
+    // LDS slot of each ccn / ccf entry, per gradient tile in chunks of four:
+    // entry k of the tile's i-th cell at ccn_base[t] + ((k/4) nt + i) 4 + k%4
+    // (nt = cells of tile t)
+    const std::uint16_t* ccn_slot;
+    const std::uint16_t* ccf_slot;
+    const int* ccn_base;      // gradient tiles + 1
+    const int* ccf_base;
+    // x_f - x_c of every cell face in ascending face index, per gradient tile:
+    // component d of face k of the tile's i-th cell at
+    // face_vec_base[t] + (d W + k) nt + i, W = the tile's longest face list
+    const double* face_vec;
+    const long long* face_vec_base;  // gradient tiles + 1
 };
 
 struct FieldArgs {

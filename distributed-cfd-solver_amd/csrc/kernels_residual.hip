@@ -190,18 +190,19 @@ __global__ void __launch_bounds__(kBlock) k_lsq_setup(MeshArgs m, FieldArgs f) {
 struct NoScalar {
     static constexpr bool kOn = false;
     __device__ double self() const { return 0.0; }
-    __device__ double nb(int) const { return 0.0; }
+    template <class Nb>
+    __device__ double nb(const Nb&) const { return 0.0; }
     __device__ void store(int, const double*) const {}
 };
 struct LdsScalar {
     static constexpr bool kOn = true;
-    const double* s;       // LDS slots
-    const std::uint16_t* slot;  // slot of each vertex-neighbour entry
+    const double* s;       // LDS slots (the neighbour's slot comes with its view)
     int own;               // slot of the cell
     double* g;             // output: 3 planes of nc
     long long nc;
     __device__ double self() const { return s[own]; }
-    __device__ double nb(int j) const { return s[slot[j]]; }
+    template <class Nb>
+    __device__ double nb(const Nb& rn) const { return s[rn.s]; }
     __device__ void store(int c, const double* v) const {
         g[c] = v[0];
         g[nc + c] = v[1];
@@ -209,9 +210,12 @@ struct LdsScalar {
     }
 };
 
-template <class Self, class NbN, class NbF, class Sc = NoScalar>
+// fvec(e, k, x0, y0, z0, ex, ey, ez) gives x_f - x_c of the cell's k-th face
+// (entry e of cf).
+template <class Self, class NbN, class NbF, class FVec, class Sc = NoScalar>
 __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
-                                                      const Self& self, NbN nbn, NbF nbf, const Sc& sc = Sc{}) {
+                                                      const Self& self, NbN nbn, NbF nbf, FVec fvec,
+                                                      const Sc& sc = Sc{}) {
     const CellRefW rc = crecw(f, c);
     double qc[5];
     for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
@@ -230,8 +234,8 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
             const CellRef ri{f.cs, f.ld, c};
             const double i0 = ri[kRInv], i1 = ri[kRInv + 1], i2 = ri[kRInv + 2], i4 = ri[kRInv + 3],
                          i5 = ri[kRInv + 4], i8 = ri[kRInv + 5];
-            for (int j = jb; j < je; ++j) {
-                const auto rn = nbn(j);
+            for (int j = jb, k = 0; j < je; ++j, ++k) {
+                const auto rn = nbn(j, k);
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double w = 1.0 / dmax(r2, 1.0e-30);
@@ -246,7 +250,7 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
                     g[v][2] += cz * d;
                 }
                 if constexpr (Sc::kOn) {
-                    const double d = sc.nb(j) - sc0;
+                    const double d = sc.nb(rn) - sc0;
                     gs[0] += cx * d;
                     gs[1] += cy * d;
                     gs[2] += cz * d;
@@ -294,8 +298,8 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
     if (!nm.muscl) return;
     double qmax[5], qmin[5];
     for (int v = 0; v < 5; ++v) qmax[v] = qmin[v] = qc[v];
-    for (int j = m.ccf_off[c]; j < m.ccf_off[c + 1]; ++j) {
-        const auto rn = nbf(j);
+    for (int j = m.ccf_off[c], k = 0; j < m.ccf_off[c + 1]; ++j, ++k) {
+        const auto rn = nbf(j, k);
         for (int v = 0; v < 5; ++v) {
             const double qn = rn[kRQ + v];
             qmax[v] = dmax(qmax[v], qn);
@@ -304,9 +308,9 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
     }
     double ph[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
     const double eps2 = nm.k3 * rc[kRVol];
-    for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const FaceRef fr = frec(m, m.cf[e]);
-        const double ex = fr[kFC] - x0, ey = fr[kFC + 1] - y0, ez = fr[kFC + 2] - z0;
+    for (int e = m.cf_off[c], k = 0; e < m.cf_off[c + 1]; ++e, ++k) {
+        double ex, ey, ez;
+        fvec(e, k, x0, y0, z0, ex, ey, ez);
         for (int v = 0; v < 5; ++v) {
             const double df = g[v][0] * ex + g[v][1] * ey + g[v][2] * ez;
             const double dnb = (df > 0.0) ? (qmax[v] - qc[v]) : (qmin[v] - qc[v]);
@@ -321,7 +325,14 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
     const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (c >= m.nc) return;
     gradient_limiter_cell(
-        m, f, nm, c, crec(f, c), [&](int j) { return crec(f, m.ccn[j]); }, [&](int j) { return crec(f, m.ccf[j]); });
+        m, f, nm, c, crec(f, c), [&](int j, int) { return crec(f, m.ccn[j]); },
+        [&](int j, int) { return crec(f, m.ccf[j]); },
+        [&](int e, int, double x0, double y0, double z0, double& ex, double& ey, double& ez) {
+            const FaceRef fr = frec(m, m.cf[e]);
+            ex = fr[kFC] - x0;
+            ey = fr[kFC + 1] - y0;
+            ez = fr[kFC + 2] - z0;
+        });
 }
 
 // View of one LDS slot of the staged gradient kernel: plane k of slot s at
@@ -344,6 +355,21 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.gtile_off[t], nt = m.gtile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
+    // slot lists in chunks of four entries (8 bytes) per cell, consecutive
+    // lanes on consecutive chunks; the first two chunks of each list are
+    // loaded before the staging so that their latency overlaps it
+    const bool live = static_cast<int>(threadIdx.x) < nt;
+    const int nchn = (m.ccn_base[t + 1] - m.ccn_base[t]) / (4 * nt);
+    const int nchf = (m.ccf_base[t + 1] - m.ccf_base[t]) / (4 * nt);
+    const uint2* chn = reinterpret_cast<const uint2*>(m.ccn_slot + m.ccn_base[t]) + threadIdx.x;
+    const uint2* chf = reinterpret_cast<const uint2*>(m.ccf_slot + m.ccf_base[t]) + threadIdx.x;
+    uint2 curn = make_uint2(0u, 0u), nxtn = curn, curf = curn, nxtf = curn;
+    if (live) {
+        if (nchn > 0) curn = chn[0];
+        if (nchn > 1) nxtn = chn[nt];
+        if (nchf > 0) curf = chf[0];
+        if (nchf > 1) nxtf = chf[nt];
+    }
     for (int sl = threadIdx.x; sl < ns; sl += blockDim.x) {
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
         for (int k = 0; k < 8; k += 2) {  // planes 0-7 = q, centroid: four 16-byte pair loads
@@ -354,15 +380,35 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
         if constexpr (kSA) sq[8 * kGradSlots + sl] = sa.nu[src];
     }
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) >= nt) return;
-    const auto nbn = [&](int j) { return LdsSlot{sq, m.ccn_slot[j]}; };
-    const auto nbf = [&](int j) { return LdsSlot{sq, m.ccf_slot[j]}; };
+    if (!live) return;
+    // entries are taken in order k = 0, 1, 2, ...: at each chunk boundary the
+    // held next chunk becomes current and the one after it is requested
+    const auto take = [&](int k, uint2& cur, uint2& nxt, const uint2* ch, int nch) {
+        if ((k & 3) == 0 && k > 0) {
+            cur = nxt;
+            if ((k >> 2) + 1 < nch) nxt = ch[((k >> 2) + 1) * nt];
+        }
+        const unsigned w = (k & 2) ? cur.y : cur.x;
+        return static_cast<int>((w >> ((k & 1) * 16)) & 0xffffu);
+    };
+    const auto nbn = [&](int, int k) { return LdsSlot{sq, take(k, curn, nxtn, chn, nchn)}; };
+    const auto nbf = [&](int, int k) { return LdsSlot{sq, take(k, curf, nxtf, chf, nchf)}; };
+    // set-up-time face vectors, transposed per tile like the slot lists
+    const long long vb = m.face_vec_base[t];
+    const long long vw = (m.face_vec_base[t + 1] - vb) / (3LL * nt);
+    const double* fvp = m.face_vec + vb + threadIdx.x;
+    const auto fvec = [&](int, int k, double, double, double, double& ex, double& ey, double& ez) {
+        ex = fvp[k * static_cast<long long>(nt)];
+        ey = fvp[(vw + k) * nt];
+        ez = fvp[(2 * vw + k) * nt];
+    };
     if constexpr (kSA) {
         // the module's nu_tilde gradient in the same pass (k_sa_gradient's arithmetic)
-        const LdsScalar sc{sq + 8 * kGradSlots, m.ccn_slot, static_cast<int>(threadIdx.x), sa.g, m.nc};
-        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, sc);
+        const LdsScalar sc{sq + 8 * kGradSlots, static_cast<int>(threadIdx.x), sa.g, m.nc};
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, fvec,
+                              sc);
     } else {
-        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf);
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, fvec);
     }
 }
 

@@ -375,8 +375,20 @@ TileStencil tile_stencil(const std::vector<i32>& tile_off, const Csr& dev_ccn, c
     const std::size_t n = dev_ccn.off.size() - 1;
     const std::vector<i32>& toff = tile_off;
     ts.halo_off.assign(ntiles + 1, 0);
-    ts.ccn_slot.assign(dev_ccn.idx.size(), 0);
-    ts.ccf_slot.assign(dev_ccf.idx.size(), 0);
+    ts.ccn_base.assign(ntiles + 1, 0);
+    ts.ccf_base.assign(ntiles + 1, 0);
+    for (std::size_t k = 0; k < ntiles; ++k) {
+        i32 wn = 0, wf = 0;
+        for (i32 c = toff[k]; c < toff[k + 1]; ++c) {
+            wn = std::max(wn, dev_ccn.off[z(c) + 1] - dev_ccn.off[z(c)]);
+            wf = std::max(wf, dev_ccf.off[z(c) + 1] - dev_ccf.off[z(c)]);
+        }
+        const i32 nt = toff[k + 1] - toff[k];
+        ts.ccn_base[k + 1] = ts.ccn_base[k] + (wn + 3) / 4 * 4 * nt;
+        ts.ccf_base[k + 1] = ts.ccf_base[k] + (wf + 3) / 4 * 4 * nt;
+    }
+    ts.ccn_slot.assign(z(ts.ccn_base[ntiles]), 0);
+    ts.ccf_slot.assign(z(ts.ccf_base[ntiles]), 0);
     std::vector<i32> slot(n, -1), local;
     for (std::size_t k = 0; k < ntiles; ++k) {
         const i32 c0 = toff[k], c1 = toff[k + 1];
@@ -398,11 +410,19 @@ TileStencil tile_stencil(const std::vector<i32>& tile_off, const Csr& dev_ccn, c
         ts.max_slots = std::max(ts.max_slots, nslots);
         for (i32 c = c0; c < c1; ++c) slot[z(c)] = c - c0;
         for (std::size_t h = 0; h < local.size(); ++h) slot[z(local[h])] = (c1 - c0) + static_cast<i32>(h);
+        const i32 nt = c1 - c0;
         for (i32 c = c0; c < c1; ++c) {
-            for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e)
-                ts.ccn_slot[z(e)] = static_cast<std::uint16_t>(slot[z(dev_ccn.idx[z(e)])]);
-            for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e)
-                ts.ccf_slot[z(e)] = static_cast<std::uint16_t>(slot[z(dev_ccf.idx[z(e)])]);
+            const i32 i = c - c0;
+            for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
+                const i32 kk = e - dev_ccn.off[z(c)];
+                ts.ccn_slot[z(ts.ccn_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] =
+                    static_cast<std::uint16_t>(slot[z(dev_ccn.idx[z(e)])]);
+            }
+            for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) {
+                const i32 kk = e - dev_ccf.off[z(c)];
+                ts.ccf_slot[z(ts.ccf_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] =
+                    static_cast<std::uint16_t>(slot[z(dev_ccf.idx[z(e)])]);
+            }
         }
         for (i32 c = c0; c < c1; ++c) slot[z(c)] = -1;
         for (const i32 h : local) slot[z(h)] = -1;

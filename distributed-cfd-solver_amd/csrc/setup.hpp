@@ -97,11 +97,20 @@ TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<std::
 // stencils reach, ascending.  ccn_slot / ccf_slot give, for every entry of the
 // device vertex-neighbour / face-neighbour lists, the neighbour's slot in the
 // LDS of the tile that owns the row.  max_slots is the largest tile + halo.
+// The slots are stored per tile in chunks of four: entry k of the list of the
+// tile's i-th cell at ccn_base[t] + ((k / 4) nt + i) 4 + k % 4 (nt = cells of
+// the tile; ccn_base[t+1] - ccn_base[t] = nt times the tile's longest list
+// rounded up to a multiple of 4; entries past a cell's own count are padding),
+// so that one 8-byte load brings four entries of a cell and the lanes of a
+// wave (the tile's consecutive cells) read consecutive chunks.  Likewise
+// ccf_slot / ccf_base.
 struct TileStencil {
     std::vector<std::int32_t> halo_off;
     std::vector<std::int32_t> halo;
     std::vector<std::uint16_t> ccn_slot;
     std::vector<std::uint16_t> ccf_slot;
+    std::vector<std::int32_t> ccn_base;
+    std::vector<std::int32_t> ccf_base;
     std::int32_t max_slots = 0;
 };
 TileStencil tile_stencil(const std::vector<std::int32_t>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf);
