@@ -1252,6 +1252,7 @@ int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulenc
         c->sa_res.alloc(z(c->nc));
         c->sa_diag.alloc(z(c->nc));
         c->sa_mdot.alloc(z(c->nf));
+        c->sa_mdot_tile.alloc(z(c->tld));
         c->sync();
         c->turb = *t;
     });

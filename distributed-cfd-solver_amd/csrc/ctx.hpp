@@ -147,6 +147,7 @@ struct cfdg_ctx {
     double sa_nu_inf = 0.0, sa_rho_nu_inf = 0.0;
     DBuf<int> sa_kind;                                  // zone of each boundary face
     DBuf<double> sa_dist, sa_rnu, sa_nu, sa_rho, sa_mut, sa_g, sa_res, sa_diag, sa_mdot;
+    DBuf<double> sa_mdot_tile;                          // mass flux per tile face-list entry
     std::vector<double> h_sa_dist;                      // wall distance, mesh order
     bool sa_on() const { return turb.model == CFDG_TURB_SA; }
     cfdg::SaArgs sargs() const {
@@ -162,6 +163,7 @@ struct cfdg_ctx {
         a.res = sa_res.get();
         a.diag = sa_diag.get();
         a.mdot = sa_mdot.get();
+        a.mdot_tile = sa_mdot_tile.get();
         return a;
     }
     // timing
@@ -213,6 +215,7 @@ struct cfdg_ctx {
         f.bflux = bflux.get();
         f.mut = sa_on() ? sa_mut.get() : nullptr;
         f.mdot = sa_on() ? sa_mdot.get() : nullptr;
+        f.mdot_tile = sa_on() ? sa_mdot_tile.get() : nullptr;
         return f;
     }
     cfdg::BsrArgs bargs() const {

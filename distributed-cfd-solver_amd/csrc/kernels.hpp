@@ -145,6 +145,7 @@ struct FieldArgs {
     double* bflux;    // F (inviscid + viscous) of every boundary face, 5 per face
     const double* mut;  // eddy viscosity (cells + ghosts, ghosts zero), null without turbulence
     double* mdot;       // mass flux F[0] of every face (mesh order), written when mut is set
+    double* mdot_tile;  // the same per tile face-list entry (tile stream order), written when mut is set
 };
 
 // Zone treatment of nu_tilde at a boundary face (spalart_allmaras.hpp:54-59)
@@ -164,6 +165,7 @@ struct SaArgs {
     double* res;          // module residual (nc)
     double* diag;         // point-implicit diagonal (nc)
     const double* mdot;   // mean-flow face mass flux (nf, mesh order)
+    const double* mdot_tile;  // the same per tile face-list entry
 };
 
 struct NumArgs {

@@ -674,7 +674,10 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             face_flux(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
                       kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
         }
-        if (kTurb && in0) f.mdot[fc[r]] = Fr[0];
+        if (kTurb) {
+            if (in0) f.mdot[fc[r]] = Fr[0];
+            f.mdot_tile[fb + j] = Fr[0];
+        }
         if (!inner)  // a boundary face lies in exactly one tile: single writer; ghost id = cells + boundary index
             for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(cy[r] - m.nc) * 5 + v] = Fr[v];
     }
