@@ -835,7 +835,10 @@ std::size_t storage_bytes(int storage) {
     return storage == cfdg::kStoreF64 ? sizeof(double) : (storage == cfdg::kStoreF32 ? sizeof(float) : sizeof(cfdg::bf16));
 }
 
-// Device-order block values as double, whatever the storage.
+// Stored values per block (bfloat16 blocks are padded, see BlockLayout).
+std::size_t block_stride(int storage) { return storage == cfdg::kStoreBF16 ? cfdg::BlockLayout<cfdg::bf16>::block : 25; }
+
+// Device-order block values as double (25 per block), whatever the storage.
 std::vector<double> download_values(cfdg_ctx& c) {
     const std::size_t n = c.blk_dev.size() * 25;
     std::vector<double> out(n);
@@ -848,10 +851,12 @@ std::vector<double> download_values(cfdg_ctx& c) {
         c.sync();
         for (std::size_t i = 0; i < n; ++i) out[i] = cfdg::to_f64(h[i]);
     } else {
-        std::vector<cfdg::bf16> h(n);
-        c.d2h(h.data(), reinterpret_cast<const cfdg::bf16*>(c.values.get()), n);
+        constexpr int kB = cfdg::BlockLayout<cfdg::bf16>::block, kR = cfdg::BlockLayout<cfdg::bf16>::row;
+        std::vector<cfdg::bf16> h(c.blk_dev.size() * kB);
+        c.d2h(h.data(), reinterpret_cast<const cfdg::bf16*>(c.values.get()), h.size());
         c.sync();
-        for (std::size_t i = 0; i < n; ++i) out[i] = cfdg::to_f64(h[i]);
+        for (std::size_t k = 0; k < c.blk_dev.size(); ++k)
+            for (int i = 0; i < 25; ++i) out[k * 25 + z(i)] = cfdg::to_f64(h[k * kB + z((i / 5) * kR + i % 5)]);
     }
     return out;
 }
@@ -867,9 +872,11 @@ void upload_values(cfdg_ctx& c, const std::vector<double>& h) {
         c.h2d(reinterpret_cast<float*>(c.values.get()), t.data(), n);
         c.sync();
     } else {
-        std::vector<cfdg::bf16> t(n);
-        for (std::size_t i = 0; i < n; ++i) t[i] = cfdg::cvt<cfdg::bf16>(h[i]);
-        c.h2d(reinterpret_cast<cfdg::bf16*>(c.values.get()), t.data(), n);
+        constexpr int kB = cfdg::BlockLayout<cfdg::bf16>::block, kR = cfdg::BlockLayout<cfdg::bf16>::row;
+        std::vector<cfdg::bf16> t(n / 25 * kB, cfdg::bf16{0});
+        for (std::size_t k = 0; k < n / 25; ++k)
+            for (int i = 0; i < 25; ++i) t[k * kB + z((i / 5) * kR + i % 5)] = cfdg::cvt<cfdg::bf16>(h[k * 25 + z(i)]);
+        c.h2d(reinterpret_cast<cfdg::bf16*>(c.values.get()), t.data(), t.size());
         c.sync();
     }
 }
@@ -886,9 +893,9 @@ void set_precision(cfdg_ctx& c, const cfdg_precision& p) {
     const int storage = p.block_storage;  // CFDG_PREC_* == kStore*
     const std::size_t nnzb = c.blk_dev.size();
     const std::size_t ncs = z(c.nc);
-    if (storage != c.storage || c.values.size() != nnzb * 25 * storage_bytes(storage)) {
+    if (storage != c.storage || c.values.size() != nnzb * block_stride(storage) * storage_bytes(storage)) {
         c.values.alloc(0);
-        c.values.alloc(nnzb * 25 * storage_bytes(storage));
+        c.values.alloc(nnzb * block_stride(storage) * storage_bytes(storage));
         c.values.zero(c.stream);
     }
     const int lu_f32 = (p.krylov == CFDG_PREC_FP32) ? 1 : 0;

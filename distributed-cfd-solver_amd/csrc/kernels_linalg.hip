@@ -42,9 +42,9 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
     // fully unrolled so that the 25 entries stay in registers: the row swap is
     // a select over the (compile-time) candidate rows
     double a[25];
-    const TS* src = values + static_cast<long long>(diag[i]) * 25;
+    const TS* src = values + static_cast<long long>(diag[i]) * BlockLayout<TS>::block;
 #pragma unroll
-    for (int k = 0; k < 25; ++k) a[k] = to_f64(src[k]);
+    for (int k = 0; k < 25; ++k) a[k] = to_f64(src[(k / 5) * BlockLayout<TS>::row + k % 5]);
     int p[5] = {0, 1, 2, 3, 4};
     bool ok = true;
 #pragma unroll
@@ -91,6 +91,26 @@ __global__ void __launch_bounds__(kBlock) k_lu_factor(int n, const int* diag, co
     for (int k = 0; k < 5; ++k) piv[static_cast<long long>(i) * 5 + k] = static_cast<std::uint8_t>(p[k]);
 }
 
+// Row ii of stored block k in the arithmetic type.  bfloat16 rows are read
+// as three 4-byte words (the row is 4-byte aligned), one wide load.
+template <class TV, class TS>
+__device__ __forceinline__ void block_row(const TS* values, long long k, int ii, TV a[5]) {
+    const TS* p = values + k * BlockLayout<TS>::block + ii * BlockLayout<TS>::row;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) a[q] = load_as<TV>(p[q]);
+}
+template <class TV>
+__device__ __forceinline__ void block_row(const bf16* values, long long k, int ii, TV a[5]) {
+    const std::uint32_t* p =
+        reinterpret_cast<const std::uint32_t*>(values + k * BlockLayout<bf16>::block + ii * BlockLayout<bf16>::row);
+    const std::uint32_t w0 = p[0], w1 = p[1], w2 = p[2];
+    a[0] = static_cast<TV>(f32_from_bits(w0 << 16));
+    a[1] = static_cast<TV>(f32_from_bits(w0 & 0xffff0000u));
+    a[2] = static_cast<TV>(f32_from_bits(w1 << 16));
+    a[3] = static_cast<TV>(f32_from_bits(w1 & 0xffff0000u));
+    a[4] = static_cast<TV>(f32_from_bits(w2 << 16));
+}
+
 // y = A x with five lanes per block row: lane ii forms
 // acc_ii = sum_k (sum_jj a_k[ii][jj] x_col(k)[jj]) in column order.
 template <class TS, class TV>
@@ -104,36 +124,40 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, cons
     TV acc = 0;
     const int kb = row_ptr[row], ke = row_ptr[row + 1];
     for (int k = kb; k < ke; ++k) {
-        const TS* a = values + static_cast<long long>(k) * 25 + ii * 5;
+        TV a[5];
+        block_row(values, k, ii, a);
         const TV* xb = x + static_cast<long long>(cols[k]) * 5;
         TV s = 0;
-        s += load_as<TV>(a[0]) * xb[0];
-        s += load_as<TV>(a[1]) * xb[1];
-        s += load_as<TV>(a[2]) * xb[2];
-        s += load_as<TV>(a[3]) * xb[3];
-        s += load_as<TV>(a[4]) * xb[4];
+        s += a[0] * xb[0];
+        s += a[1] * xb[1];
+        s += a[2] * xb[2];
+        s += a[3] * xb[3];
+        s += a[4] * xb[4];
         acc += s;
     }
     y[row * 5 + ii] = acc;
 }
 
 // One Gauss-Seidel update for each row of a group: z_i = LU_i^-1 (r_i - sum_{k != d} A_ik z_k).
-// Eight lanes per row (five active): lane ii forms component ii of the sum over
-// the row's off-diagonal blocks, reading block row ii (5 contiguous values), so
-// the five lanes of a row read each 5x5 block as one contiguous 25-value run;
-// the five sums are then exchanged within the group of eight and every lane
-// performs the same 5x5 LU solve (lu_solve_block<5>, bsr_helpers.hpp:325-353)
-// and keeps its component.  The arithmetic and its order per component are
-// those of one thread per row.
+// Five lanes per row, twelve rows per wave (lanes 60-63 idle): lane ii forms
+// component ii of the sum over the row's off-diagonal blocks, reading block
+// row ii (5 contiguous values), so the five lanes of a row read each 5x5 block
+// as one contiguous run; the five sums are then exchanged within the row's
+// five lanes and every lane performs the same 5x5 LU solve
+// (lu_solve_block<5>, bsr_helpers.hpp:325-353) and keeps its component.  The
+// arithmetic and its order per component are those of one thread per row.
+constexpr int kSgsRowsPerWave = 12;
 template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
                                                       const int* diag, const TS* values, const TL* lu,
                                                       const std::uint8_t* piv, const TV* r, TV* z, const double* gate) {
     if (gate && gate[kKsStatus] != 0.0) return;
-    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long long slot = t >> 3;
-    const int ii = static_cast<int>(t & 7);
-    const bool has_row = slot < count;  // uniform over the group of eight
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int rw = lane / 5;  // row of the wave, 12 = idle lanes
+    const int ii = lane - rw * 5;
+    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const long long slot = wave * kSgsRowsPerWave + rw;
+    const bool has_row = rw < kSgsRowsPerWave && slot < count;  // uniform over the row's five lanes
     const int i = has_row ? rows[slot] : 0;
     TV acc = 0;
     if (has_row && ii < 5) {
@@ -142,20 +166,21 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
         const int kb = row_ptr[i], ke = row_ptr[i + 1];
         for (int k = kb; k < ke; ++k) {
             if (k == d) continue;
-            const TS* a = values + static_cast<long long>(k) * 25 + ii * 5;
+            TV a[5];
+            block_row(values, k, ii, a);
             const TV* zb = z + static_cast<long long>(cols[k]) * 5;
             TV tt = 0;
-            tt += load_as<TV>(a[0]) * zb[0];
-            tt += load_as<TV>(a[1]) * zb[1];
-            tt += load_as<TV>(a[2]) * zb[2];
-            tt += load_as<TV>(a[3]) * zb[3];
-            tt += load_as<TV>(a[4]) * zb[4];
+            tt += a[0] * zb[0];
+            tt += a[1] * zb[1];
+            tt += a[2] * zb[2];
+            tt += a[3] * zb[3];
+            tt += a[4] * zb[4];
             acc -= tt;
         }
     }
-    const int base = static_cast<int>(threadIdx.x & 63) & ~7;
+    const int base = rw * 5;  // idle lanes (rw = 12) read lanes 60-63 and discard
     TV s[5];
-    for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, base + q, 64);
+    for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, (base + q) & 63, 64);
     if (!has_row || ii >= 5) return;
     const TL* L = lu + static_cast<long long>(i) * 25;
     const std::uint8_t* p = piv + static_cast<long long>(i) * 5;
@@ -529,7 +554,7 @@ template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
                       const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate) {
     if (count <= 0) return;
-    const dim3 g = grid_for(8LL * count), blk(kBlock);
+    const dim3 g = grid_for((static_cast<long long>(count) + kSgsRowsPerWave - 1) / kSgsRowsPerWave * 64), blk(kBlock);
     if constexpr (sizeof(TV) == sizeof(double)) {
         if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
         hipLaunchKernelGGL((k_sgs_group<double, double, double>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,

@@ -721,6 +721,29 @@ __global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cs, long lo
     alpha[c] = (l * inv_cfl) + (vol * 0.0);
 }
 
+// Store one 5x5 block (values v, sign +1 or -1 applied as 0.0 + v / 0.0 - v,
+// or as is) in the storage layout; bfloat16 rows go out as three 4-byte words.
+template <class TS>
+__device__ __forceinline__ void store_block(TS* dst, const double v[25], int sign) {
+    for (int i = 0; i < 25; ++i) {
+        const double x = sign > 0 ? 0.0 + v[i] : (sign < 0 ? 0.0 - v[i] : v[i]);
+        dst[(i / 5) * BlockLayout<TS>::row + i % 5] = cvt<TS>(x);
+    }
+}
+template <>
+__device__ __forceinline__ void store_block<bf16>(bf16* dst, const double v[25], int sign) {
+    std::uint32_t* w = reinterpret_cast<std::uint32_t*>(dst);
+    for (int a = 0; a < 5; ++a) {
+        std::uint32_t h[6];
+        for (int q = 0; q < 5; ++q) {
+            const double x = sign > 0 ? 0.0 + v[a * 5 + q] : (sign < 0 ? 0.0 - v[a * 5 + q] : v[a * 5 + q]);
+            h[q] = cvt<bf16>(x).u;
+        }
+        h[5] = 0;
+        for (int q = 0; q < 3; ++q) w[a * 3 + q] = h[2 * q] | (h[2 * q + 1] << 16);
+    }
+}
+
 __device__ __forceinline__ int find_col(const BsrArgs& b, int row, int col) {
     int k = b.row_ptr[row];
     const int e = b.row_ptr[row + 1];
@@ -782,18 +805,13 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, Nu
                     for (int i = 0; i < 25; ++i) dg[i] -= J[i];
             } else {
                 const int k = find_col(b, c, own ? c1 : c0);
-                TS* dst = values + static_cast<long long>(k) * 25;
-                if (own)
-                    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(0.0 + J[i]);
-                else
-                    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(0.0 - J[i]);
+                store_block(values + static_cast<long long>(k) * BlockLayout<TS>::block, J, own ? 1 : -1);
             }
         }
     }
     const double a = alpha[c];
     for (int i = 0; i < 5; ++i) dg[i * 6] += a;
-    TS* dst = values + static_cast<long long>(b.diag[c]) * 25;
-    for (int i = 0; i < 25; ++i) dst[i] = cvt<TS>(dg[i]);
+    store_block(values + static_cast<long long>(b.diag[c]) * BlockLayout<TS>::block, dg, 0);
 }
 
 // Per-patch boundary integrals (residual_kernel.hpp:96-169 forms F[0] and

@@ -47,6 +47,19 @@ __host__ __device__ __forceinline__ float bf16_to_float(bf16 h) {
     return f32_from_bits(static_cast<std::uint32_t>(h.u) << 16);
 }
 
+// Layout of one stored 5x5 block: entry (a, b) at a * row + b of a block of
+// `block` values.  bfloat16 rows are padded to 6 values (12 bytes, 4-byte
+// aligned) in 64-byte blocks, so that a block row is one 12-byte load rather
+// than five 2-byte loads.
+template <class TS>
+struct BlockLayout {
+    static constexpr int block = 25, row = 5;
+};
+template <>
+struct BlockLayout<bf16> {
+    static constexpr int block = 32, row = 6;
+};
+
 // Conversions between the storage and arithmetic types.
 template <class To>
 __host__ __device__ __forceinline__ To cvt(double v);
