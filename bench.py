@@ -247,8 +247,9 @@ def sa_residual(a):
 def time_to_drop(a):
     """Wall time to a 1e-8 drop of the reference's relative residual
     (max_v L2_v / L2_v of the first step, solver.hpp:740-765) on a steady
-    laminar plate, for the plain FP64 path, the FP32-Krylov form and the FP32
-    form with the learned warm start.  Second-order unlimited MUSCL with the
+    laminar plate, for the plain FP64 path, the two mixed forms (FP32 or BF16
+    blocks, FP32 Krylov inside FP64 defect correction) and the BF16 form with
+    the learned warm start.  Second-order unlimited MUSCL with the
     node-LSQ gradient (the Venkatakrishnan limiter stalls this case near 1e-6),
     CFL 1000, from the freestream with seeded 2 % noise.  The network is
     trained first on 8 seeded 32x16x8 plates with Mach drawn from U(0.2, 0.5)
@@ -264,7 +265,8 @@ def time_to_drop(a):
     ni, nj, nk = (int(x) for x in a.drop_size.split("x"))
     kw = dict(limiter="UNLIMITED", cfl=1000.0)
     lp = cases.default_linparams()
-    fp32 = cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1)
+    forms = {"fp32": cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1),
+             "bf16": cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1)}
 
     def start(case, seed):
         mesh = cfdg.Mesh.generate(**case.mesh)
@@ -278,7 +280,7 @@ def time_to_drop(a):
         case = cases.plate(ni=ni, nj=nj, nk=nk, **kw)
         s = start(case, 1)
         if form != "plain":
-            s.set_precision(fp32)
+            s.set_precision(forms[form])
         if gnn is not None:
             s.set_warm_start(gnn)
         r = s.run_steady(lp, a.drop_max_steps, rel_tol=1e-8)
@@ -294,10 +296,11 @@ def time_to_drop(a):
                        "CFL 1000, BiCGSTAB + multicolour SGS (rel 0.1), 2% seeded noise on the freestream"}
     out["plain_fp64"], U0 = run("plain")
     out["fp32_krylov"], U1 = run("fp32")
+    out["bf16_blocks_fp32_krylov"], U3 = run("bf16")
     if a.drop_previous_update:
         zero = cfdg.Gnn(seed=1)
         zero.set_params(np.zeros(zero.num_params, np.float32))
-        out["fp32_krylov_previous_update"], _ = run("learned", zero)
+        out["bf16_blocks_previous_update"], _ = run("bf16", zero)
     t0 = time.perf_counter()
     gnn = cfdg.Gnn(seed=11)
     rng = np.random.default_rng(2024)
@@ -313,12 +316,15 @@ def time_to_drop(a):
     loss = gnn.train(epochs=30, lr=1e-3, seed=5)
     out["training"] = {"seconds": round(time.perf_counter() - t0, 2), "samples": gnn.num_samples(),
                        "final_loss": loss}
-    out["fp32_krylov_learned"], U2 = run("learned", gnn)
+    out["bf16_blocks_fp32_krylov_learned"], U2 = run("bf16", gnn)
     scale = np.abs(U0).max(axis=1, keepdims=True)
     out["state_error_vs_fp64"] = {"fp32_krylov": float(np.max(np.abs(U1 - U0) / scale)),
-                                  "fp32_krylov_learned": float(np.max(np.abs(U2 - U0) / scale))}
-    out["speedup_mixed_learned"] = round(out["plain_fp64"]["seconds"] / out["fp32_krylov_learned"]["seconds"], 3)
-    out["speedup_mixed"] = round(out["plain_fp64"]["seconds"] / out["fp32_krylov"]["seconds"], 3)
+                                  "bf16_blocks_fp32_krylov": float(np.max(np.abs(U3 - U0) / scale)),
+                                  "bf16_blocks_fp32_krylov_learned": float(np.max(np.abs(U2 - U0) / scale))}
+    t0 = out["plain_fp64"]["seconds"]
+    out["speedup_mixed_fp32"] = round(t0 / out["fp32_krylov"]["seconds"], 3)
+    out["speedup_mixed_bf16"] = round(t0 / out["bf16_blocks_fp32_krylov"]["seconds"], 3)
+    out["speedup_mixed_learned"] = round(t0 / out["bf16_blocks_fp32_krylov_learned"]["seconds"], 3)
     return out
 
 

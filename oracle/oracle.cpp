@@ -422,6 +422,12 @@ struct oracle_ctx {
         std::vector<double> dist, rnu, nu, rho, mut, g, res, mdot, diag;
     } sa;
     bool sa_on() const { return sa.cfg.model == 1; }
+    // dual time stepping (solver.hpp:355-380): BDF order 0 = steady, else the
+    // physical time step and the stored time levels U^n, U^{n-1} (5 planes of nc)
+    int bdf_order = 0, bdf_cur = 1;
+    int64_t phys_step = 0;  // physical steps taken since dual time was set
+    double phys_dt = 0.0;
+    std::vector<double> Un, Unm1;
 };
 
 namespace {
@@ -1393,6 +1399,25 @@ void sa_update(oracle_ctx& c) {
 }
 
 // solver.hpp:169-247
+// BDF pseudo-source R += V (c0 U + c1 U^n [+ c2 U^{n-1}]) (solver.hpp:546-575)
+void add_unsteady_source(oracle_ctx& c) {
+    const double inv_dt = 1.0 / c.phys_dt;
+    const size_t n = static_cast<size_t>(c.nc);
+    for (int v = 0; v < 5; ++v) {
+        const double* u = plane(c.U, c.ntot, v);
+        const double* un = c.Un.data() + static_cast<size_t>(v) * n;
+        const double* unm1 = c.Unm1.data() + static_cast<size_t>(v) * n;
+        double* r = plane(c.res, c.ntot, v);
+        if (c.bdf_cur == 2) {
+            const double c0 = 1.5 * inv_dt, c1 = -2.0 * inv_dt, c2 = 0.5 * inv_dt;
+            for (size_t i = 0; i < n; ++i) r[i] += c.vol[i] * (c0 * u[i] + c1 * un[i] + c2 * unm1[i]);
+        } else {
+            const double c0 = 1.0 * inv_dt, c1 = -1.0 * inv_dt;
+            for (size_t i = 0; i < n; ++i) r[i] += c.vol[i] * (c0 * u[i] + c1 * un[i]);
+        }
+    }
+}
+
 void evaluate_residual(oracle_ctx& c) {
     for (i32 i = 0; i < c.nc; ++i) {
         const size_t is = static_cast<size_t>(i);
@@ -1420,17 +1445,32 @@ void evaluate_residual(oracle_ctx& c) {
         sa_face_sweep(c);
         sa_cell_sources(c);
     }
+    if (c.bdf_order > 0) add_unsteady_source(c);
     c.have_residual = true;
 }
 
-// solver.hpp:256-287 (implicit branch, steady)
+// solver.hpp:256-287, implicit branch; the unsteady diagonal term V c0 / dt
+// with c0 = 1.5 (BDF2) or 1 (BDF1) in dual time stepping
 void compute_dt(oracle_ctx& c) {
     const double inv_cfl = 1.0 / c.num.cfl;
+    double unsteady = 0.0;
+    if (c.bdf_order > 0) unsteady = ((c.bdf_cur == 2) ? 1.5 : 1.0) / c.phys_dt;
     for (i32 i = 0; i < c.nc; ++i) {
         const size_t is = static_cast<size_t>(i);
         const double l = std::max(c.lam[is], kLamFloor);
         c.dt[is] = c.num.cfl * c.vol[is] / l;
-        c.alpha[is] = (l * inv_cfl) + (c.vol[is] * 0.0);
+        c.alpha[is] = (l * inv_cfl) + (c.vol[is] * unsteady);
+    }
+}
+
+// solver.hpp:256-287, explicit branch: alpha = dt / V = CFL / lambda
+void compute_dt_explicit(oracle_ctx& c) {
+    for (i32 i = 0; i < c.nc; ++i) {
+        const size_t is = static_cast<size_t>(i);
+        const double l = std::max(c.lam[is], kLamFloor);
+        const double cfl_over_l = c.num.cfl / l;
+        c.dt[is] = cfl_over_l * c.vol[is];
+        c.alpha[is] = cfl_over_l;
     }
 }
 
@@ -1994,6 +2034,94 @@ int oracle_implicit_step(oracle_ctx* c, const oracle_linparams* lp, oracle_step_
         c->err = e.what();
         return -1;
     }
+}
+
+// ForwardEuler::advance (forward_euler.hpp:24-30): residual, explicit dt,
+// U <- U - alpha R (update_ops.hpp:28-44); l2 = norms of that residual.
+int oracle_explicit_step(oracle_ctx* c, double l2[5]) {
+    if (!c) return -1;
+    try {
+        if (c->sa_on()) throw std::runtime_error("explicit stepping with the turbulence module is not supported");
+        if (c->bdf_order > 0) throw std::runtime_error("explicit stepping in dual time is not supported");
+        evaluate_residual(*c);
+        compute_dt_explicit(*c);
+        if (l2) oracle_residual_norms(c, l2);
+        for (int v = 0; v < 5; ++v) {
+            double* u = plane(c->U, c->ntot, v);
+            const double* r = plane(c->res, c->ntot, v);
+            for (i32 i = 0; i < c->nc; ++i) u[i] = u[i] - c->alpha[static_cast<size_t>(i)] * r[i];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return -1;
+    }
+}
+
+// Dual time stepping on (order 1 or 2; 0 = steady): U^n = U^{n-1} = the
+// current state, as Solver::init_fields sets them (solver.hpp:494-528).
+int oracle_set_unsteady(oracle_ctx* c, double dt, int32_t order) {
+    if (!c) return -1;
+    if (order < 0 || order > 2 || (order > 0 && !(dt > 0.0))) {
+        c->err = "unsteady: order 0, 1 or 2 and dt > 0 required";
+        return -1;
+    }
+    c->bdf_order = order;
+    c->phys_dt = dt;
+    c->bdf_cur = 1;
+    c->phys_step = 0;
+    const size_t n = static_cast<size_t>(c->nc);
+    c->Un.assign(5 * n, 0.0);
+    for (int v = 0; v < 5; ++v) std::copy_n(plane(c->U, c->ntot, v), n, c->Un.data() + static_cast<size_t>(v) * n);
+    c->Unm1 = c->Un;
+    return 0;
+}
+
+// Solver::run_dual_time (solver.hpp:662-736): per physical step the BDF order
+// (1 on the first step since dual time was set; the count carries over
+// between calls), implicit sub-iterations with the relative residual checked
+// at sub-iteration 1, every 5th and the last, then the time levels shift.
+// subiters receives the sub-iterations of every physical step.
+int oracle_run_unsteady(oracle_ctx* c, const oracle_linparams* lp, int64_t steps, int64_t max_subiter, double sub_tol,
+                        int64_t* subiters) {
+    if (!c || !lp) return -1;
+    if (c->bdf_order == 0) {
+        c->err = "unsteady: not enabled";
+        return -1;
+    }
+    const size_t n = static_cast<size_t>(c->nc);
+    for (int64_t k = 0; k < steps; ++k) {
+        const int64_t step = ++c->phys_step;  // counts on across calls
+        c->bdf_cur = (step == 1 || c->bdf_order == 1) ? 1 : 2;
+        double norm0[5] = {0, 0, 0, 0, 0};
+        bool have0 = false;
+        int64_t last = 0;
+        for (int64_t it = 1; it <= max_subiter; ++it) {
+            last = it;
+            oracle_step_result st{};
+            if (oracle_implicit_step(c, lp, &st) != 0) return -1;
+            const bool diagnose = it == 1 || it % 5 == 0 || it == max_subiter;
+            if (!diagnose) continue;
+            if (!std::isfinite(st.l2[0])) {
+                c->err = "unsteady: non-finite residual";
+                return -1;
+            }
+            double rel = 1.0;
+            if (!have0) {
+                for (int v = 0; v < 5; ++v) norm0[v] = st.l2[v];
+                have0 = true;
+            } else {
+                rel = 0.0;
+                for (int v = 0; v < 5; ++v) rel = std::max(rel, st.l2[v] / std::max(norm0[v], 1.0e-300));
+            }
+            if (rel <= sub_tol) break;
+        }
+        if (subiters) subiters[k] = last;
+        // U^{n-1} = U^n, U^n = U^{n+1}
+        if (c->bdf_order == 2) std::swap(c->Unm1, c->Un);
+        for (int v = 0; v < 5; ++v) std::copy_n(plane(c->U, c->ntot, v), n, c->Un.data() + static_cast<size_t>(v) * n);
+    }
+    return 0;
 }
 
 int oracle_row_groups(oracle_ctx* c, int32_t* colour, int32_t* n_colours, int32_t* fwd, int32_t* n_fwd, int32_t* bwd,

@@ -104,6 +104,13 @@ int oracle_set_bsr_values(oracle_ctx* ctx, const double* values);
 int oracle_linear_solve(oracle_ctx* ctx, const oracle_linparams* lp, const double* b, const double* x0,
                         double* x, oracle_iter_result* res);
 int oracle_implicit_step(oracle_ctx* ctx, const oracle_linparams* lp, oracle_step_result* out);
+/* forward Euler step with the explicit dt (forward_euler.hpp:24-30); l2 of the residual before the update */
+int oracle_explicit_step(oracle_ctx* ctx, double l2[5]);
+/* dual time stepping: order 0 (steady), 1 or 2; U^n = U^{n-1} = the current state */
+int oracle_set_unsteady(oracle_ctx* ctx, double dt, int32_t order);
+/* Solver::run_dual_time (solver.hpp:662-736); subiters[steps] receives the sub-iterations per physical step */
+int oracle_run_unsteady(oracle_ctx* ctx, const oracle_linparams* lp, int64_t steps, int64_t max_subiter,
+                        double sub_tol, int64_t* subiters);
 
 /* Greedy colouring of the face graph in row order (colour = smallest colour
    not used by an already-coloured lower neighbour) and the dependency levels

@@ -41,6 +41,8 @@ def _load():
         "oracle_set_turbulence": [vp, vp], "oracle_set_turbulence_state": [vp, vp],
         "oracle_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "oracle_sa_point": [d, d, d, d, d, d, vp],
+        "oracle_explicit_step": [vp, vp], "oracle_set_unsteady": [vp, d, i32],
+        "oracle_run_unsteady": [vp, vp, i64, i64, d, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -177,6 +179,21 @@ class Oracle:
         out = StepResult()
         self._chk(lib.oracle_implicit_step(self._h, C.byref(lp), C.byref(out)))
         return out
+
+    def explicit_step(self):
+        """One forward Euler step; returns the L2 norms of the residual before the update."""
+        l2 = np.zeros(5)
+        self._chk(lib.oracle_explicit_step(self._h, _ptr(l2)))
+        return l2
+
+    def set_unsteady(self, dt, order):
+        self._chk(lib.oracle_set_unsteady(self._h, float(dt), int(order)))
+
+    def run_unsteady(self, lp, steps, max_subiter, sub_tol):
+        """Dual time stepping; returns the sub-iterations of every physical step."""
+        sub = np.zeros(int(steps), np.int64)
+        self._chk(lib.oracle_run_unsteady(self._h, C.byref(lp), int(steps), int(max_subiter), float(sub_tol), _ptr(sub)))
+        return sub
 
     def row_groups(self):
         n = self.n_cells

@@ -43,6 +43,7 @@
 #include "cfd/solver/compressible/physics/viscous_flow.hpp"
 #include "cfd/solver/compressible/residual_kernel.hpp"
 #include "cfd/solver/compressible/solver.hpp"
+#include "cfd/solver/compressible/time/forward_euler.hpp"
 #include "cfd/solver/compressible/time/implicit_euler.hpp"
 #include "cfd/solver/time_mode.hpp"
 
@@ -398,6 +399,74 @@ void run_solver_path(const Recs& r, const mesh::MeshPart& mp, const bc::Boundary
     }
 }
 
+// Time stepping through Solver::run from the uniform initial state: N forward
+// Euler iterations (forward_euler.hpp) and/or the dual-time BDF1/BDF2 driver
+// (solver.hpp:662-736, BackwardEuler inner iterations).
+template <typename Phys, typename Recon>
+void run_time_paths(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfig& bcfg, const Phys& phys,
+                    Writer& w) {
+    using EOS = sc::eos::IdealGas;
+    const EOS eos{scalar<double>(r, "gamma"), scalar<double>(r, "gas_constant")};
+    sc::SolverConfig cfg;
+    cfg.flow.gamma = eos.gamma_val;
+    cfg.flow.gas_constant = eos.R_val;
+    cfg.prandtl = scalar<double>(r, "prandtl");
+    cfg.flow_model = Phys::kHasViscous ? sc::FlowModel::ViscousFlow : sc::FlowModel::InviscidFlow;
+    const auto init = get<double>(r, "init");  // rho, u, v, w, p
+    cfg.init_rho = init[0];
+    cfg.init_velocity = {init[1], init[2], init[3]};
+    cfg.init_p = init[4];
+    cfg.gradient = grad_type(scalar<std::int32_t>(r, "gradient"));
+    cfg.limiter_venkat_k = scalar<double>(r, "venkat_k");
+    cfg.field_interval = 0;
+    cfg.residual_interval = 1;
+    cfg.output_dir = "/tmp/cfdg_ref_out";
+    cfg.linear_solver_params.type = linalg::SolverType::BICGSTAB;
+    cfg.linear_solver_params.precond_type =
+        scalar<std::int32_t>(r, "precond") == 1 ? linalg::PreconditionerType::SGS : linalg::PreconditionerType::None;
+    cfg.linear_solver_params.relative_tolerance = scalar<double>(r, "rel_tol");
+    cfg.linear_solver_params.absolute_tolerance = scalar<double>(r, "abs_tol");
+    cfg.linear_solver_params.max_iterations = scalar<std::int32_t>(r, "max_iter");
+    const std::size_t nc = static_cast<std::size_t>(mp.n_cells);
+    const auto dump = [&](const char* name, auto& solver) {
+        std::vector<double> uo;
+        const auto u = solver.u_slots();
+        for (int v = 0; v < 5; ++v) uo.insert(uo.end(), u[static_cast<std::size_t>(v)], u[static_cast<std::size_t>(v)] + nc);
+        w.put(name, uo);
+    };
+    if (r.count("explicit_steps") && scalar<std::int32_t>(r, "explicit_steps") > 0) {
+        sc::SolverConfig c = cfg;
+        c.scheme = sc::TimeScheme::ForwardEuler;
+        c.cfl = scalar<double>(r, "explicit_cfl");
+        c.max_iterations = scalar<std::int32_t>(r, "explicit_steps");
+        c.residual_tolerance = 1e-300;
+        sc::Solver<EOS, sc::fluxes::HllcFlux, Recon, Phys, sc::time::ForwardEuler, solver::SteadyMode> s(
+            c, bcfg, eos, phys, mp, MPI_COMM_WORLD);
+        s.run();
+        dump("explicit_U", s);
+    }
+    if (r.count("unsteady_steps") && scalar<std::int32_t>(r, "unsteady_steps") > 0) {
+        sc::SolverConfig c = cfg;
+        c.scheme = sc::TimeScheme::BackwardEuler;
+        c.cfl = scalar<double>(r, "cfl");
+        c.dt = scalar<double>(r, "unsteady_dt");
+        c.max_time_steps = scalar<std::int32_t>(r, "unsteady_steps");
+        c.max_iterations = scalar<std::int32_t>(r, "unsteady_subiter");
+        c.residual_tolerance = scalar<double>(r, "unsteady_sub_tol");
+        if (scalar<std::int32_t>(r, "unsteady_order") == 1) {
+            sc::Solver<EOS, sc::fluxes::HllcFlux, Recon, Phys, sc::time::BackwardEuler, solver::UnsteadyMode<1>> s(
+                c, bcfg, eos, phys, mp, MPI_COMM_WORLD);
+            s.run();
+            dump("unsteady_U", s);
+        } else {
+            sc::Solver<EOS, sc::fluxes::HllcFlux, Recon, Phys, sc::time::BackwardEuler, solver::UnsteadyMode<2>> s(
+                c, bcfg, eos, phys, mp, MPI_COMM_WORLD);
+            s.run();
+            dump("unsteady_U", s);
+        }
+    }
+}
+
 // Wall time of Solver::evaluate_residual on the given state (MPI_Wtime around
 // `reps` calls), the reference's own CPU cost of one residual evaluation.
 template <typename Phys, typename Recon>
@@ -442,6 +511,7 @@ void run_all(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfig& 
     }
     run_components<Phys, Recon>(r, mp, bcfg, phys, w);
     if (scalar<std::int32_t>(r, "solver_path") != 0) run_solver_path<Phys, Recon>(r, mp, bcfg, phys, w);
+    run_time_paths<Phys, Recon>(r, mp, bcfg, phys, w);
 }
 
 template <typename Phys>

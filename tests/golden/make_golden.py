@@ -6,9 +6,11 @@ Each fixture holds the inputs (mesh arrays, numerics, boundary conditions,
 state, linear-solver parameters) and the reference's outputs: primitives with
 ghosts, gradients, limiters, residual, spectral radius, local time step,
 diagonal shift, BSR pattern and Jacobian blocks, one BiCGSTAB+SGS solution,
-and the state after a few implicit steps from the freestream.
+and the state after a few implicit steps from the freestream.  The time_*
+fixtures hold the state after forward Euler iterations and after a few
+physical steps of dual time stepping (BDF1, BDF2) from the uniform state.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [time]
 """
 import os
 import sys
@@ -106,6 +108,52 @@ def make(name, case, steps, seed=11):
     print(f"{name}: {nc} cells -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+def time_configs():
+    """(name, case, explicit CFL, explicit steps, physical dt, physical steps, sub-iterations, sub tolerance)."""
+    return [
+        ("time_ramp", cases.ramp(ni=12, nj=4, nk=2), 0.5, 25, 1.0e-4, 3, 12, 1.0e-3),
+        ("time_plate", cases.plate(ni=8, nj=6, nk=4), 0.5, 25, 1.0e-4, 3, 12, 1.0e-3),
+    ]
+
+
+def make_time(name, case, cfl_explicit, n_explicit, dt, n_phys, n_sub, sub_tol):
+    """Forward Euler (forward_euler.hpp) and dual-time BDF1 / BDF2 runs of the
+    reference's Solver from the uniform initial state."""
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    arrays = mesh.arrays()
+    rho, (u, v, w), p = case.rho, case.velocity, case.p
+    U = np.ones((5, mesh.n_cells))
+    lp = case.linparams
+    lp.sgs_order = cfdg.SGS_SEQUENTIAL
+    lp.rel_tol = 1e-3
+    fx = {f"mesh_{k}": a for k, a in arrays.items()}
+    fx["n_patches"] = np.array([mesh.n_patches])
+    fx.update({f"num_{k}": a for k, a in numerics_dict(case.numerics).items()})
+    fx.update(bcs_dict(case.bcs))
+    fx["init"] = np.array([rho, u, v, w, p])
+    fx["lin"] = np.array([lp.preconditioner, lp.sgs_order, lp.rel_tol, lp.abs_tol, lp.max_iter], dtype=np.float64)
+    fx["explicit"] = np.array([cfl_explicit, n_explicit])
+    fx["unsteady"] = np.array([dt, n_phys, n_sub, sub_tol])
+    for order in (1, 2):
+        rec = ref_run.case_records(arrays, mesh.n_patches, case.numerics, case.bcs, U, [rho, u, v, w, p], lp,
+                                   steps=0, solver_path=0)
+        if order == 2:
+            rec["explicit_steps"] = np.array([n_explicit], np.int32)
+            rec["explicit_cfl"] = np.array([cfl_explicit])
+        rec["unsteady_steps"] = np.array([n_phys], np.int32)
+        rec["unsteady_order"] = np.array([order], np.int32)
+        rec["unsteady_dt"] = np.array([dt])
+        rec["unsteady_subiter"] = np.array([n_sub], np.int32)
+        rec["unsteady_sub_tol"] = np.array([sub_tol])
+        out = ref_run.run(rec)
+        fx[f"ref_bdf{order}_U"] = out["unsteady_U"]
+        if order == 2:
+            fx["ref_explicit_U"] = out["explicit_U"]
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **fx)
+    print(f"{name}: {mesh.n_cells} cells -> {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def make_rcm():
     """Reference renumbering of a natural-order mesh (reorder.cpp:135-188, 306-576)."""
     case = cases.plate(ni=10, nj=6, nk=4, reorder=0)
@@ -125,6 +173,10 @@ def make_rcm():
 if __name__ == "__main__":
     if not ref_run.available():
         sys.exit("reference harness missing: run oracle/build_ref.sh first")
-    for name, case, steps in configs():
-        make(name, case, steps)
-    make_rcm()
+    only_time = len(sys.argv) > 1 and sys.argv[1] == "time"
+    if not only_time:
+        for name, case, steps in configs():
+            make(name, case, steps)
+        make_rcm()
+    for args in time_configs():
+        make_time(*args)
