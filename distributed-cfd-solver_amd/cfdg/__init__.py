@@ -159,6 +159,8 @@ def _load():
         "cfdg_assemble_jacobian": [vp], "cfdg_get_bsr_pattern": [vp, vp, vp, vp],
         "cfdg_get_bsr_values": [vp, vp], "cfdg_set_bsr_values": [vp, vp],
         "cfdg_linear_solve": [vp, vp, vp, vp, vp, vp], "cfdg_implicit_step": [vp, vp, vp],
+        "cfdg_explicit_step": [vp, vp], "cfdg_set_unsteady": [vp, _d, _i32],
+        "cfdg_run_unsteady": [vp, vp, _i64, _i64, _d, vp],
         "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
         "cfdg_set_precision": [vp, vp], "cfdg_get_precision": [vp, vp], "cfdg_run_steady": [vp, vp, vp, vp],
         "cfdg_boundary_integrals": [vp, vp, vp, vp, _i32],
@@ -411,6 +413,24 @@ class Solver:
         out = StepResult()
         self._chk(lib.cfdg_implicit_step(self._h, C.byref(lp), C.byref(out)))
         return out
+
+    def explicit_step(self):
+        """One forward Euler step (CFL of the numerics); returns the StepResult
+        with the norms of the residual before the update."""
+        out = StepResult()
+        self._chk(lib.cfdg_explicit_step(self._h, C.byref(out)))
+        return out
+
+    def set_unsteady(self, dt, order):
+        """Dual time stepping with BDF order 1 or 2 and physical step dt (order 0: steady)."""
+        self._chk(lib.cfdg_set_unsteady(self._h, float(dt), int(order)))
+
+    def run_unsteady(self, lp, steps, max_subiter, sub_tol):
+        """Physical steps of dual time stepping; returns the sub-iterations of each."""
+        sub = np.zeros(int(steps), np.int64)
+        self._chk(lib.cfdg_run_unsteady(self._h, C.byref(lp), int(steps), int(max_subiter), float(sub_tol),
+                                        sub.ctypes.data_as(C.c_void_p)))
+        return sub
 
     def boundary_integrals(self):
         """Per patch: net mass flux, energy flux, force (momentum flux, 3) of the last residual."""

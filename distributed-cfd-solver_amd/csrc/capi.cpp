@@ -185,6 +185,9 @@ void eval_residual(cfdg_ctx& c) {
     cfdg::launch_flux(m, f, c.nargs, c.stream);
     // module face sweep on the recorded mass flux, then its sources (solver.hpp:233-241)
     if (c.sa_on()) cfdg::launch_sa_residual(m, f, c.nargs, c.sargs(), c.stream);
+    // BDF pseudo-source of dual time stepping (solver.hpp:243-246)
+    if (c.bdf_order > 0)
+        cfdg::launch_unsteady_source(m, f, c.bdf_cur, 1.0 / c.phys_dt, c.Un.get(), c.Unm1.get(), c.stream);
     hip_check(hipEventRecord(c.ev[4], c.stream), "event");
     c.check_launch();
     c.sync();
@@ -198,7 +201,8 @@ void eval_residual(cfdg_ctx& c) {
 
 void compute_dt(cfdg_ctx& c) {
     if (!c.have_residual) throw CfdgError(CFDG_ERR_STATE, "compute_dt before eval_residual");
-    cfdg::launch_dt(c.margs(), c.fargs(), c.lam.get(), c.num.cfl, c.dt.get(), c.alpha.get(), c.stream);
+    const double unsteady = (c.bdf_order > 0) ? ((c.bdf_cur == 2) ? 1.5 : 1.0) / c.phys_dt : 0.0;
+    cfdg::launch_dt(c.margs(), c.fargs(), c.lam.get(), c.num.cfl, unsteady, c.dt.get(), c.alpha.get(), c.stream);
     c.check_launch();
     c.sync();
     c.have_dt = true;
@@ -1177,6 +1181,93 @@ int cfdg_implicit_step(cfdg_ctx* c, const cfdg_linparams* lp, cfdg_step_result* 
     });
 }
 
+int cfdg_explicit_step(cfdg_ctx* c, cfdg_step_result* out) {
+    return guarded(c, [&] {
+        if (c->sa_on()) throw CfdgError(CFDG_ERR_CONFIG, "explicit stepping with the turbulence module is not supported");
+        if (c->bdf_order > 0) throw CfdgError(CFDG_ERR_STATE, "explicit stepping in dual time is not supported");
+        eval_residual(*c);
+        cfdg::launch_dt_explicit(c->margs(), c->fargs(), c->lam.get(), c->num.cfl, c->dt.get(), c->alpha.get(),
+                                 c->stream);
+        c->have_dt = false;  // alpha holds the explicit dt / V, not the implicit diagonal shift
+        double l2[5];
+        residual_norms(*c, l2);
+        cfdg::launch_explicit_update(c->margs(), c->fargs(), c->alpha.get(), c->U.get(), c->stream);
+        c->check_launch();
+        c->sync();
+        c->have_residual = false;
+        if (out) {
+            *out = cfdg_step_result{};
+            for (int v = 0; v < 5; ++v) out->l2[v] = l2[v];
+        }
+    });
+}
+
+int cfdg_set_unsteady(cfdg_ctx* c, double dt, int32_t order) {
+    return guarded(c, [&] {
+        if (order < 0 || order > 2 || (order > 0 && !(dt > 0.0)))
+            throw CfdgError(CFDG_ERR_ARG, "dual time stepping needs order 0, 1 or 2 and dt > 0");
+        if (order > 0 && c->sa_on())
+            throw CfdgError(CFDG_ERR_CONFIG, "dual time stepping with the turbulence module is not supported");
+        c->bdf_order = order;
+        c->phys_dt = dt;
+        c->bdf_cur = 1;
+        c->phys_step = 0;
+        const std::size_t n = 5 * z(c->nc);
+        if (order > 0) {
+            if (c->Un.size() != n) c->Un.alloc(n);
+            if (c->Unm1.size() != n) c->Unm1.alloc(n);
+            hip_check(hipMemcpyAsync(c->Un.get(), c->U.get(), n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                      "copy");
+            hip_check(hipMemcpyAsync(c->Unm1.get(), c->U.get(), n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                      "copy");
+            c->sync();
+        }
+        c->have_residual = false;
+        c->have_dt = false;
+    });
+}
+
+int cfdg_run_unsteady(cfdg_ctx* c, const cfdg_linparams* lp, int64_t steps, int64_t max_subiter, double sub_tol,
+                      int64_t* subiters) {
+    return guarded(c, [&] {
+        if (!lp) throw CfdgError(CFDG_ERR_ARG, "null linear parameters");
+        if (c->bdf_order == 0) throw CfdgError(CFDG_ERR_STATE, "dual time stepping is not enabled");
+        if (steps < 0 || max_subiter < 1 || !(sub_tol >= 0.0))
+            throw CfdgError(CFDG_ERR_ARG, "steps >= 0, max_subiter >= 1 and sub_tol >= 0 required");
+        const std::size_t n = 5 * z(c->nc);
+        for (int64_t k = 0; k < steps; ++k) {
+            const int64_t step = ++c->phys_step;  // counts on across calls
+            c->bdf_cur = (step == 1 || c->bdf_order == 1) ? 1 : 2;
+            double norm0[5] = {0, 0, 0, 0, 0};
+            bool have0 = false;
+            int64_t last = 0;
+            for (int64_t it = 1; it <= max_subiter; ++it) {
+                last = it;
+                cfdg_step_result st{};
+                implicit_step(*c, *lp, &st);
+                const bool diagnose = it == 1 || it % 5 == 0 || it == max_subiter;
+                if (!diagnose) continue;
+                if (!std::isfinite(st.l2[0])) throw CfdgError(CFDG_ERR_STATE, "non-finite residual in dual time stepping");
+                double rel = 1.0;
+                if (!have0) {
+                    for (int v = 0; v < 5; ++v) norm0[v] = st.l2[v];
+                    have0 = true;
+                } else {
+                    rel = 0.0;
+                    for (int v = 0; v < 5; ++v) rel = std::max(rel, st.l2[v] / std::max(norm0[v], 1.0e-300));
+                }
+                if (rel <= sub_tol) break;
+            }
+            if (subiters) subiters[k] = last;
+            // U^{n-1} = U^n, U^n = U^{n+1}
+            if (c->bdf_order == 2) c->Unm1.swap(c->Un);
+            hip_check(hipMemcpyAsync(c->Un.get(), c->U.get(), n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                      "copy");
+            c->sync();
+        }
+    });
+}
+
 int cfdg_boundary_integrals(cfdg_ctx* c, double* mass, double* energy, double* force, int32_t n_patches) {
     return guarded(c, [&] {
         if (!c->have_residual) throw CfdgError(CFDG_ERR_STATE, "boundary_integrals needs an evaluated residual");
@@ -1199,6 +1290,8 @@ int cfdg_boundary_integrals(cfdg_ctx* c, double* mass, double* energy, double* f
 int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulence* t) {
     return guarded(c, [&] {
         if (!t) throw CfdgError(CFDG_ERR_ARG, "null turbulence parameters");
+        if (t->model != CFDG_TURB_NONE && c->bdf_order > 0)
+            throw CfdgError(CFDG_ERR_CONFIG, "the turbulence module is not supported in dual time stepping");
         c->have_residual = false;
         c->have_dt = false;
         if (t->model == CFDG_TURB_NONE) {

@@ -43,6 +43,10 @@ class DBuf {
     }
     T* get() const { return p_; }
     std::size_t size() const { return n_; }
+    void swap(DBuf& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
 
   private:
     void release() {
@@ -142,6 +146,12 @@ struct cfdg_ctx {
     bool features_had_prev = false;     // du_prev was valid for the current features
     bool have_features = false;
     long long warm_used = 0, warm_rejected = 0;
+    // dual time stepping (off while bdf_order == 0): physical time step, the
+    // BDF order of the current physical step and the time levels U^n, U^{n-1}
+    int bdf_order = 0, bdf_cur = 1;
+    int64_t phys_step = 0;
+    double phys_dt = 0.0;
+    DBuf<double> Un, Unm1;
     // Spalart-Allmaras module (off unless turb.model == CFDG_TURB_SA)
     cfdg_turbulence turb{};
     double sa_nu_inf = 0.0, sa_rho_nu_inf = 0.0;

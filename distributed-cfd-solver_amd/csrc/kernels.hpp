@@ -211,8 +211,16 @@ void launch_ghost_gradients(const MeshArgs& m, const FieldArgs& f, const NumArgs
 void launch_flux(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, hipStream_t s);
 // out[5 p + v] = sum of F[v] over the boundary faces of patch p, p < n_patches
 void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patches, double* out, hipStream_t s);
-void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
-               hipStream_t s);
+// unsteady = c0 / dt of dual time stepping (0 in steady mode)
+void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double unsteady, double* dt,
+               double* alpha, hipStream_t s);
+// forward Euler: alpha = CFL / lambda, then U <- U - alpha R
+void launch_dt_explicit(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt,
+                        double* alpha, hipStream_t s);
+void launch_explicit_update(const MeshArgs& m, const FieldArgs& f, const double* alpha, double* U, hipStream_t s);
+// dual time stepping: R += V (c0 U + c1 U^n [+ c2 U^{n-1}]), bdf = 1 or 2
+void launch_unsteady_source(const MeshArgs& m, const FieldArgs& f, int bdf, double inv_dt, const double* Un,
+                            const double* Unm1, hipStream_t s);
 // values: nnzb x 25 entries of the storage type (kStoreF64 / kStoreF32 / kStoreBF16)
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
                      const double* alpha, void* values, int storage, hipStream_t s);

@@ -709,16 +709,59 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
     __builtin_nontemporal_store(lam, f.lam + c);
 }
 
-// Implicit local time step and diagonal shift (solver.hpp:256-287)
+// Implicit local time step and diagonal shift (solver.hpp:256-287); unsteady
+// = c0 / dt of dual time stepping (0 in steady mode)
 __global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cs, long long ld, const double* lam, double cfl,
-                                               double* dt, double* alpha) {
+                                               double unsteady, double* dt, double* alpha) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const double vol = cs[plane_at(kRVol, ld, c)];
     const double inv_cfl = 1.0 / cfl;
     const double l = dmax(lam[c], 1.0e-14);
     dt[c] = cfl * vol / l;
-    alpha[c] = (l * inv_cfl) + (vol * 0.0);
+    alpha[c] = (l * inv_cfl) + (vol * unsteady);
+}
+
+// Explicit local time step: alpha = dt / V = CFL / lambda (solver.hpp:278-285)
+__global__ void __launch_bounds__(kBlock) k_dt_explicit(int nc, const double* cs, long long ld, const double* lam,
+                                                        double cfl, double* dt, double* alpha) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double l = dmax(lam[c], 1.0e-14);
+    const double cfl_over_l = cfl / l;
+    dt[c] = cfl_over_l * cs[plane_at(kRVol, ld, c)];
+    alpha[c] = cfl_over_l;
+}
+
+// Forward Euler update U <- U - alpha R (update_ops.hpp:28-44)
+__global__ void __launch_bounds__(kBlock) k_explicit_update(int nc, const double* alpha, const double* res, double* U) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double a = alpha[c];
+    for (int v = 0; v < 5; ++v) {
+        const long long k = static_cast<long long>(v) * nc + c;
+        U[k] = U[k] - a * res[k];
+    }
+}
+
+// BDF pseudo-source of dual time stepping, R += V (c0 U + c1 U^n [+ c2 U^{n-1}])
+// (solver.hpp:546-575); U^n, U^{n-1} as 5 planes of nc
+__global__ void __launch_bounds__(kBlock) k_unsteady_source(int nc, const double* cs, long long ld, int bdf,
+                                                            double inv_dt, const double* U, const double* Un,
+                                                            const double* Unm1, double* res) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double vol = cs[plane_at(kRVol, ld, c)];
+    for (int v = 0; v < 5; ++v) {
+        const long long k = static_cast<long long>(v) * nc + c;
+        if (bdf == 2) {
+            const double c0 = 1.5 * inv_dt, c1 = -2.0 * inv_dt, c2 = 0.5 * inv_dt;
+            res[k] += vol * (c0 * U[k] + c1 * Un[k] + c2 * Unm1[k]);
+        } else {
+            const double c0 = 1.0 * inv_dt, c1 = -1.0 * inv_dt;
+            res[k] += vol * (c0 * U[k] + c1 * Un[k]);
+        }
+    }
 }
 
 // Store one 5x5 block (values v, sign +1 or -1 applied as 0.0 + v / 0.0 - v,
@@ -892,9 +935,27 @@ void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patc
                            m.nf - m.ninner, m.bface_bc, f.bflux, out);
 }
 
-void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt, double* alpha,
-               hipStream_t s) {
-    if (m.nc > 0) hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.cs, f.ld, lam, cfl, dt, alpha);
+void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double unsteady, double* dt,
+               double* alpha, hipStream_t s) {
+    if (m.nc > 0)
+        hipLaunchKernelGGL(k_dt, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.cs, f.ld, lam, cfl, unsteady, dt, alpha);
+}
+
+void launch_dt_explicit(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt,
+                        double* alpha, hipStream_t s) {
+    if (m.nc > 0)
+        hipLaunchKernelGGL(k_dt_explicit, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.cs, f.ld, lam, cfl, dt, alpha);
+}
+
+void launch_explicit_update(const MeshArgs& m, const FieldArgs& f, const double* alpha, double* U, hipStream_t s) {
+    if (m.nc > 0) hipLaunchKernelGGL(k_explicit_update, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, alpha, f.res, U);
+}
+
+void launch_unsteady_source(const MeshArgs& m, const FieldArgs& f, int bdf, double inv_dt, const double* Un,
+                            const double* Unm1, hipStream_t s) {
+    if (m.nc > 0)
+        hipLaunchKernelGGL(k_unsteady_source, grid_for(m.nc), dim3(kBlock), 0, s, m.nc, f.cs, f.ld, bdf, inv_dt, f.U,
+                           Un, Unm1, f.res);
 }
 
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,

@@ -277,6 +277,35 @@ typedef struct {
 int cfdg_run_steady(cfdg_ctx* ctx, const cfdg_linparams* lp, const cfdg_steady_params* sp,
                     cfdg_steady_result* out);
 
+/* ---- Time integration other than the implicit pseudo-time step --------------
+ *
+ * cfdg_explicit_step   ForwardEuler::advance (forward_euler.hpp:24-30): the
+ *                      residual, the explicit local step alpha = CFL / lambda
+ *                      (solver.hpp:278-285, CFL of the numerics), then
+ *                      U <- U - alpha R (update_ops.hpp:28-44).  out->l2 = the
+ *                      norms of that residual; out->linear is zero.  Not with
+ *                      the turbulence module or in dual time (CFDG_ERR_CONFIG /
+ *                      CFDG_ERR_STATE).
+ * cfdg_set_unsteady    dual time stepping (solver.hpp:355-380, 494-528): BDF
+ *                      order 1 or 2 with physical step dt, or 0 for steady; the
+ *                      time levels U^n = U^{n-1} = the current state and the
+ *                      physical step count restarts.  Every residual then adds
+ *                      V (c0 U + c1 U^n [+ c2 U^{n-1}]) (solver.hpp:546-575) and
+ *                      the implicit diagonal shift adds V c0 / dt, c0 = 1.5
+ *                      (BDF2) or 1 (BDF1, and the first physical step).
+ * cfdg_run_unsteady    Solver::run_dual_time (solver.hpp:662-736): `steps`
+ *                      physical steps, each of implicit sub-iterations until
+ *                      the relative residual (checked at sub-iteration 1, every
+ *                      5th and the last) reaches sub_tol or max_subiter runs
+ *                      out, then U^{n-1} = U^n, U^n = U.  subiters[steps], if
+ *                      given, receives the sub-iterations of each step.  The
+ *                      physical step count carries over between calls.
+ */
+int cfdg_explicit_step(cfdg_ctx* ctx, cfdg_step_result* out);
+int cfdg_set_unsteady(cfdg_ctx* ctx, double dt, int32_t order);
+int cfdg_run_unsteady(cfdg_ctx* ctx, const cfdg_linparams* lp, int64_t steps, int64_t max_subiter, double sub_tol,
+                      int64_t* subiters);
+
 /* ---- precision forms ------------------------------------------------------
    No counterpart in the reference, whose linear algebra is FP64 throughout
    (BsrMatrix values are double, bsr_matrix.hpp:24-62).  The default form is

@@ -2066,6 +2066,10 @@ int oracle_set_unsteady(oracle_ctx* c, double dt, int32_t order) {
         c->err = "unsteady: order 0, 1 or 2 and dt > 0 required";
         return -1;
     }
+    if (order > 0 && c->sa_on()) {
+        c->err = "dual time stepping with the turbulence module is not supported";
+        return -1;
+    }
     c->bdf_order = order;
     c->phys_dt = dt;
     c->bdf_cur = 1;
@@ -2217,6 +2221,10 @@ int oracle_wall_distance(oracle_ctx* c, int32_t stencil, int32_t max_sweeps, dou
 
 int oracle_set_turbulence(oracle_ctx* c, const oracle_turbulence* t) {
     if (!c || !t) return -1;
+    if (t->model != 0 && c->bdf_order > 0) {
+        c->err = "the turbulence module is not supported in dual time stepping";
+        return -1;
+    }
     if (t->model == 0) {
         c->sa.cfg = *t;
         return 0;
