@@ -671,6 +671,15 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
             std::memcpy(&tph[z(cfdg::plane_at(cfdg::kTCells, c.tld, e))], &ids, sizeof ids);
         }
         c.tf.upload(tph, s);
+        // entry of every face in the face list of the tile that holds its owner
+        std::vector<i32> fentry(z(c.nf), -1);
+        for (int k = 0; k < c.ntiles; ++k)
+            for (i32 e = tf.tile_face_off[z(k)]; e < tf.tile_face_off[z(k) + 1]; ++e) {
+                const i32 f = tf.faces[z(e)];
+                const i32 o = owner_dev[z(f)];
+                if (o >= tl.tile_off[z(k)] && o < tl.tile_off[z(k) + 1]) fentry[z(f)] = e;
+            }
+        c.face_entry.upload(fentry, s);
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
         c.ccf_off.upload(ccf_dev.off, s);
         c.ccf.upload(ccf_dev.idx, s);
@@ -1391,6 +1400,8 @@ int cfdg_get_turbulence(cfdg_ctx* c, double* rho_nu, double* nu, double* mut, do
         if (grad)
             for (int d = 0; d < 3; ++d) cells(c->sa_g, grad + static_cast<long long>(d) * c->nc, static_cast<long long>(d) * c->nc);
         if (mdot) {
+            cfdg::launch_gather_mdot(c->nf, c->face_entry.get(), c->sa_mdot_tile.get(), c->sa_mdot.get(), c->stream);
+            c->check_launch();
             c->d2h(mdot, c->sa_mdot.get(), z(c->nf));
             c->sync();
         }

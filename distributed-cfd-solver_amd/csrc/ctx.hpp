@@ -158,6 +158,7 @@ struct cfdg_ctx {
     DBuf<int> sa_kind;                                  // zone of each boundary face
     DBuf<double> sa_dist, sa_rnu, sa_nu, sa_rho, sa_mut, sa_g, sa_res, sa_diag, sa_mdot;
     DBuf<double> sa_mdot_tile;                          // mass flux per tile face-list entry
+    DBuf<int> face_entry;                               // entry of each face in its owner tile's face list
     std::vector<double> h_sa_dist;                      // wall distance, mesh order
     bool sa_on() const { return turb.model == CFDG_TURB_SA; }
     cfdg::SaArgs sargs() const {

@@ -144,8 +144,8 @@ struct FieldArgs {
     double* lam;      // nc
     double* bflux;    // F (inviscid + viscous) of every boundary face, 5 per face
     const double* mut;  // eddy viscosity (cells + ghosts, ghosts zero), null without turbulence
-    double* mdot;       // mass flux F[0] of every face (mesh order), written when mut is set
-    double* mdot_tile;  // the same per tile face-list entry (tile stream order), written when mut is set
+    double* mdot;       // mass flux F[0] of every face (mesh order), gathered from mdot_tile on request
+    double* mdot_tile;  // mass flux per tile face-list entry (tile stream order), written when mut is set
 };
 
 // Zone treatment of nu_tilde at a boundary face (spalart_allmaras.hpp:54-59)
@@ -214,6 +214,8 @@ void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patc
 // unsteady = c0 / dt of dual time stepping (0 in steady mode)
 void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double unsteady, double* dt,
                double* alpha, hipStream_t s);
+// mdot[f] = mdot_tile[face_entry[f]] for every face (mesh order)
+void launch_gather_mdot(int nf, const int* face_entry, const double* mdot_tile, double* mdot, hipStream_t s);
 // forward Euler: alpha = CFL / lambda, then U <- U - alpha R
 void launch_dt_explicit(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double* dt,
                         double* alpha, hipStream_t s);

@@ -614,12 +614,12 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
     const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
     const int fb = m.tile_face_off[t], nfl = m.tile_face_off[t + 1] - fb;
     const int i = threadIdx.x;
-    int fc[kPerLane], cx[kPerLane], cy[kPerLane];
+    int cx[kPerLane], cy[kPerLane];
     double fz[kPerLane];
 #pragma unroll
     for (int r = 0; r < kPerLane; ++r) {  // this lane's faces: cells (and index), issued before the staging loads
         const int j = i + r * kFluxThreads;
-        fc[r] = cx[r] = cy[r] = 0;
+        cx[r] = cy[r] = 0;
         fz[r] = 0.0;
         if (j < nfl) {
             typedef double pair_t __attribute__((ext_vector_type(2)));
@@ -629,7 +629,6 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             fz[r] = w.x;
             cx[r] = static_cast<int>(static_cast<unsigned>(ids));
             cy[r] = static_cast<int>(static_cast<unsigned>(ids >> 32));
-            if (kTurb) fc[r] = __builtin_nontemporal_load(m.tile_faces + fb + j);
         }
     }
     // staging of the plane pairs the face fluxes read: q (pairs 0-2), centroid
@@ -674,10 +673,7 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             face_flux(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
                       kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
         }
-        if (kTurb) {
-            if (in0) f.mdot[fc[r]] = Fr[0];
-            f.mdot_tile[fb + j] = Fr[0];
-        }
+        if (kTurb) f.mdot_tile[fb + j] = Fr[0];
         if (!inner)  // a boundary face lies in exactly one tile: single writer; ghost id = cells + boundary index
             for (int v = 0; v < 5; ++v) f.bflux[static_cast<long long>(cy[r] - m.nc) * 5 + v] = Fr[v];
     }
@@ -720,6 +716,15 @@ __global__ void __launch_bounds__(kBlock) k_dt(int nc, const double* cs, long lo
     const double l = dmax(lam[c], 1.0e-14);
     dt[c] = cfl * vol / l;
     alpha[c] = (l * inv_cfl) + (vol * unsteady);
+}
+
+// Mesh-order face mass flux from the tile-order copy: entry of each face in
+// the face list of the tile that holds its owner
+__global__ void __launch_bounds__(kBlock) k_gather_mdot(int nf, const int* face_entry, const double* mdot_tile,
+                                                        double* mdot) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    mdot[f] = mdot_tile[face_entry[f]];
 }
 
 // Explicit local time step: alpha = dt / V = CFL / lambda (solver.hpp:278-285)
@@ -933,6 +938,10 @@ void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patc
     if (n_patches > 0)
         hipLaunchKernelGGL(k_boundary_integrals, dim3(static_cast<unsigned>(n_patches)), dim3(kBlock), 0, s,
                            m.nf - m.ninner, m.bface_bc, f.bflux, out);
+}
+
+void launch_gather_mdot(int nf, const int* face_entry, const double* mdot_tile, double* mdot, hipStream_t s) {
+    if (nf > 0) hipLaunchKernelGGL(k_gather_mdot, grid_for(nf), dim3(kBlock), 0, s, nf, face_entry, mdot_tile, mdot);
 }
 
 void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double unsteady, double* dt,
