@@ -161,6 +161,7 @@ def _load():
         "cfdg_linear_solve": [vp, vp, vp, vp, vp, vp], "cfdg_implicit_step": [vp, vp, vp],
         "cfdg_explicit_step": [vp, vp], "cfdg_set_unsteady": [vp, _d, _i32],
         "cfdg_run_unsteady": [vp, vp, _i64, _i64, _d, vp],
+        "cfdg_write_solution_vtu": [vp, vp, C.c_char_p, C.c_char_p],
         "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
         "cfdg_set_precision": [vp, vp], "cfdg_get_precision": [vp, vp], "cfdg_run_steady": [vp, vp, vp, vp],
         "cfdg_boundary_integrals": [vp, vp, vp, vp, _i32],
@@ -431,6 +432,10 @@ class Solver:
         self._chk(lib.cfdg_run_unsteady(self._h, C.byref(lp), int(steps), int(max_subiter), float(sub_tol),
                                         sub.ctypes.data_as(C.c_void_p)))
         return sub
+
+    def write_solution_vtu(self, outdir, stem):
+        """Writes <outdir>/<stem>_00000.vtu and <outdir>/<stem>.pvtu as the reference's solver does."""
+        self._chk(lib.cfdg_write_solution_vtu(self._h, self.mesh._h, str(outdir).encode(), str(stem).encode()))
 
     def boundary_integrals(self):
         """Per patch: net mass flux, energy flux, force (momentum flux, 3) of the last residual."""

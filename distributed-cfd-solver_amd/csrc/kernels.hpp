@@ -214,6 +214,8 @@ void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patc
 // unsteady = c0 / dt of dual time stepping (0 in steady mode)
 void launch_dt(const MeshArgs& m, const FieldArgs& f, const double* lam, double cfl, double unsteady, double* dt,
                double* alpha, hipStream_t s);
+// rho, u, v, w, p, Mach of every cell (6 planes of nc, device order) for the solution files
+void launch_output_fields(int nc, const Gas& g, const double* U, double* out, hipStream_t s);
 // mdot[f] = mdot_tile[face_entry[f]] for every face (mesh order)
 void launch_gather_mdot(int nf, const int* face_entry, const double* mdot_tile, double* mdot, hipStream_t s);
 // forward Euler: alpha = CFL / lambda, then U <- U - alpha R

@@ -727,6 +727,29 @@ __global__ void __launch_bounds__(kBlock) k_gather_mdot(int nf, const int* face_
     mdot[f] = mdot_tile[face_entry[f]];
 }
 
+// Output fields of the solution files (Solver::write_fields, solver.hpp:856-878):
+// rho, u, v, w, p = (gamma - 1) rho e (state_conversions.hpp:12-25), Mach =
+// |rho v| / (rho a) with a = sqrt(gamma p / rho); out = 6 planes of nc
+__global__ void __launch_bounds__(kBlock) k_output_fields(int nc, Gas g, const double* U, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double u[5];
+    for (int v = 0; v < 5; ++v) u[v] = U[static_cast<long long>(v) * nc + c];
+    const double r = u[0];
+    const double inv_rho = 1.0 / r;
+    const double q2 = (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * (inv_rho * inv_rho);
+    const double e = (u[4] * inv_rho) - 0.5 * q2;
+    const double p = (g.gamma - 1.0) * r * e;
+    const double a = sqrt(g.gamma * p / r);
+    double* o = out + c;
+    o[0] = r;
+    o[static_cast<long long>(nc)] = u[1] / r;
+    o[2LL * nc] = u[2] / r;
+    o[3LL * nc] = u[3] / r;
+    o[4LL * nc] = p;
+    o[5LL * nc] = sqrt(u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / (r * a);
+}
+
 // Explicit local time step: alpha = dt / V = CFL / lambda (solver.hpp:278-285)
 __global__ void __launch_bounds__(kBlock) k_dt_explicit(int nc, const double* cs, long long ld, const double* lam,
                                                         double cfl, double* dt, double* alpha) {
@@ -938,6 +961,10 @@ void launch_boundary_integrals(const MeshArgs& m, const FieldArgs& f, int n_patc
     if (n_patches > 0)
         hipLaunchKernelGGL(k_boundary_integrals, dim3(static_cast<unsigned>(n_patches)), dim3(kBlock), 0, s,
                            m.nf - m.ninner, m.bface_bc, f.bflux, out);
+}
+
+void launch_output_fields(int nc, const Gas& g, const double* U, double* out, hipStream_t s) {
+    if (nc > 0) hipLaunchKernelGGL(k_output_fields, grid_for(nc), dim3(kBlock), 0, s, nc, g, U, out);
 }
 
 void launch_gather_mdot(int nf, const int* face_entry, const double* mdot_tile, double* mdot, hipStream_t s) {

@@ -302,6 +302,18 @@ int cfdg_run_steady(cfdg_ctx* ctx, const cfdg_linparams* lp, const cfdg_steady_p
  *                      physical step count carries over between calls.
  */
 int cfdg_explicit_step(cfdg_ctx* ctx, cfdg_step_result* out);
+/* ---- Solution output ---------------------------------------------------------
+ *
+ * cfdg_write_solution_vtu  Solver::write_fields (solver.hpp:856-898) with
+ *                          io::vtk::write_solution_vtu (vtu.cpp:321-475) for
+ *                          one rank: <outdir>/<stem>_00000.vtu (the mesh, a
+ *                          rank array and the cell fields rho, u, v, w,
+ *                          pressure, mach of the current state, formed on the
+ *                          device) and its master <outdir>/<stem>.pvtu, in the
+ *                          reference's text formats; outdir is created if
+ *                          needed.  mesh = the mesh the context was created on.
+ */
+int cfdg_write_solution_vtu(cfdg_ctx* ctx, const cfdg_mesh* mesh, const char* outdir, const char* stem);
 int cfdg_set_unsteady(cfdg_ctx* ctx, double dt, int32_t order);
 int cfdg_run_unsteady(cfdg_ctx* ctx, const cfdg_linparams* lp, int64_t steps, int64_t max_subiter, double sub_tol,
                       int64_t* subiters);

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <span>
 #include <stdexcept>
@@ -104,7 +105,10 @@ struct Writer {
         const std::uint32_t ln = static_cast<std::uint32_t>(n.size());
         f.write(reinterpret_cast<const char*>(&ln), 4);
         f.write(n.data(), ln);
-        unsigned char code = std::is_same_v<T, double> ? 0 : (std::is_same_v<T, std::int32_t> ? 1 : 3);
+        unsigned char code = std::is_same_v<T, double>         ? 0
+                             : std::is_same_v<T, std::int32_t> ? 1
+                             : std::is_same_v<T, std::uint8_t> ? 2
+                                                               : 3;
         const std::uint64_t c = count;
         f.write(reinterpret_cast<const char*>(&code), 1);
         f.write(reinterpret_cast<const char*>(&c), 8);
@@ -396,6 +400,15 @@ void run_solver_path(const Recs& r, const mesh::MeshPart& mp, const bc::Boundary
         const auto u2 = s2.u_slots();
         for (int v = 0; v < 5; ++v) uo.insert(uo.end(), u2[static_cast<std::size_t>(v)], u2[static_cast<std::size_t>(v)] + nc);
         w.put("steps_U", uo);
+        // the solution files run() wrote at the end (Solver::write_fields, vtu.cpp:321-424)
+        char stem[96];
+        std::snprintf(stem, sizeof(stem), "%s/final_%09lld", cfg.output_dir.c_str(), static_cast<long long>(steps));
+        const auto slurp = [](const std::string& path) {
+            std::ifstream in(path, std::ios::binary);
+            return std::vector<std::uint8_t>((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        };
+        w.put("vtu_piece", slurp(std::string(stem) + "_00000.vtu"));
+        w.put("vtu_master", slurp(std::string(stem) + ".pvtu"));
     }
 }
 

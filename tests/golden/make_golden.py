@@ -8,9 +8,10 @@ ghosts, gradients, limiters, residual, spectral radius, local time step,
 diagonal shift, BSR pattern and Jacobian blocks, one BiCGSTAB+SGS solution,
 and the state after a few implicit steps from the freestream.  The time_*
 fixtures hold the state after forward Euler iterations and after a few
-physical steps of dual time stepping (BDF1, BDF2) from the uniform state.
+physical steps of dual time stepping (BDF1, BDF2) from the uniform state;
+the vtu_* fixtures the reference's solution files after a few implicit steps.
 
-usage: python tests/golden/make_golden.py [time]
+usage: python tests/golden/make_golden.py [time|vtu]
 """
 import os
 import sys
@@ -154,6 +155,33 @@ def make_time(name, case, cfl_explicit, n_explicit, dt, n_phys, n_sub, sub_tol):
     print(f"{name}: {mesh.n_cells} cells -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+def make_vtu(name, case, steps, seed=11):
+    """The reference's solution files (Solver::write_fields after run(),
+    vtu.cpp:321-424) with the state they hold."""
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    arrays = mesh.arrays()
+    rho, (u, v, w), p = case.rho, case.velocity, case.p
+    U = np.ones((5, mesh.n_cells))
+    lp = case.linparams
+    lp.sgs_order = cfdg.SGS_SEQUENTIAL
+    lp.rel_tol = 1e-3
+    rec = ref_run.case_records(arrays, mesh.n_patches, case.numerics, case.bcs, U, [rho, u, v, w, p], lp, steps=steps)
+    out = ref_run.run(rec)
+    fx = {f"mesh_{k}": a for k, a in arrays.items()}
+    fx["n_patches"] = np.array([mesh.n_patches])
+    fx.update({f"num_{k}": a for k, a in numerics_dict(case.numerics).items()})
+    fx.update(bcs_dict(case.bcs))
+    fx["init"] = np.array([rho, u, v, w, p])
+    fx["lin"] = np.array([lp.preconditioner, lp.sgs_order, lp.rel_tol, lp.abs_tol, lp.max_iter], dtype=np.float64)
+    fx["steps"] = np.array([steps])
+    fx["ref_steps_U"] = out["steps_U"]
+    fx["ref_vtu_piece"] = out["vtu_piece"]
+    fx["ref_vtu_master"] = out["vtu_master"]
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **fx)
+    print(f"{name}: {mesh.n_cells} cells, VTU {len(out['vtu_piece'])} bytes -> {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def make_rcm():
     """Reference renumbering of a natural-order mesh (reorder.cpp:135-188, 306-576)."""
     case = cases.plate(ni=10, nj=6, nk=4, reorder=0)
@@ -173,10 +201,14 @@ def make_rcm():
 if __name__ == "__main__":
     if not ref_run.available():
         sys.exit("reference harness missing: run oracle/build_ref.sh first")
-    only_time = len(sys.argv) > 1 and sys.argv[1] == "time"
-    if not only_time:
+    only = sys.argv[1] if len(sys.argv) > 1 else ""
+    if not only:
         for name, case, steps in configs():
             make(name, case, steps)
         make_rcm()
-    for args in time_configs():
-        make_time(*args)
+    if only in ("", "time"):
+        for args in time_configs():
+            make_time(*args)
+    if only in ("", "vtu"):
+        make_vtu("vtu_ramp", cases.ramp(ni=12, nj=4, nk=2), 3)
+        make_vtu("vtu_bump", cases.bump(ni=3, nj=3, nk=3), 2)
