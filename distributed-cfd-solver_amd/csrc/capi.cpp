@@ -693,15 +693,17 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.ccn_off.upload(st.off, s);
         c.ccn.upload(st.idx, s);
         // LDS-staged stencils: the flux tiles when their cells + halo fit the
-        // kernel's slots, else consecutive sub-ranges of them that do
+        // kernel's slots (the wide form when the narrow one does not hold
+        // them), else consecutive sub-ranges of them that fit the wide form
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
         std::vector<i32> goff = tl.tile_off;
         cfdg::TileStencil ts = cfdg::tile_stencil(goff, st, ccf_dev);
-        if (ts.max_slots > cfdg::kGradSlots) {
-            goff = cfdg::split_tiles(tl.tile_off, st, ccf_dev, cfdg::kGradSlots);
+        if (ts.max_slots > cfdg::kGradSlotsBig) {
+            goff = cfdg::split_tiles(tl.tile_off, st, ccf_dev, cfdg::kGradSlotsBig);
             ts = cfdg::tile_stencil(goff, st, ccf_dev);
         }
-        if (ts.max_slots <= cfdg::kGradSlots) {
+        c.gslots = (ts.max_slots <= cfdg::kGradSlots) ? cfdg::kGradSlots : cfdg::kGradSlotsBig;
+        if (ts.max_slots <= cfdg::kGradSlotsBig) {
             int most = 0;
             for (std::size_t k = 0; k + 1 < goff.size(); ++k) most = std::max(most, goff[k + 1] - goff[k]);
             c.ngtiles = static_cast<int>(goff.size()) - 1;

@@ -93,7 +93,7 @@ struct cfdg_ctx {
     long long fld = 0, tld = 0;
     DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
     DBuf<int> tile_halo_off, tile_halo, gtile_off;  // gradient tiles (the flux tiles or sub-ranges of them)
-    int ngtiles = 0, gblock = 0;
+    int ngtiles = 0, gblock = 0, gslots = 0;
     DBuf<std::uint16_t> ccn_slot, ccf_slot;
     DBuf<int> ccn_base, ccf_base;
     DBuf<double> face_vec;        // limiter face vectors per gradient tile (transposed)
@@ -207,6 +207,7 @@ struct cfdg_ctx {
         a.gtile_off = gtile_off.get();
         a.ngtiles = ngtiles;
         a.gblock = gblock;
+        a.gslots = gslots;
         a.tile_halo = tile_halo.get();
         a.ccn_slot = ccn_slot.get();
         a.ccf_slot = ccf_slot.get();

@@ -22,6 +22,7 @@ constexpr int kTileFaces = 1024;      // faces per tile held in LDS by the flux 
 constexpr int kFluxThreads = 512;     // threads per flux workgroup (kTileFaces / kFluxThreads faces per lane)
 constexpr int kNumXcd = 8;            // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kGradSlots = 640;       // LDS slots (tile cells + stencil halo) of the staged gradient kernel
+constexpr int kGradSlotsBig = 1088;   // its wide form, for large stencils (tetrahedra): 2 workgroups per CU
 // storage of the Jacobian blocks (values of the precision enumeration of cfdg.h)
 constexpr int kStoreF64 = 0, kStoreF32 = 1, kStoreBF16 = 2;
 
@@ -120,6 +121,7 @@ struct MeshArgs {
     const int* gtile_off;     // gradient tiles + 1: the flux tiles, or consecutive sub-ranges of them
     int ngtiles;
     int gblock;               // threads per gradient workgroup (multiple of 64, >= cells of any gradient tile)
+    int gslots;               // LDS slot capacity of the gradient kernel: kGradSlots or kGradSlotsBig
     const int* tile_halo_off; // gradient tiles + 1
     const int* tile_halo;     // halo cells of each gradient tile, ascending device id
     // LDS slot of each ccn / ccf entry, per gradient tile in chunks of four:

@@ -336,11 +336,12 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
 }
 
 // View of one LDS slot of the staged gradient kernel: plane k of slot s at
-// sq[k * kGradSlots + s].
+// sq[k * kSlots + s].
+template <int kSlots>
 struct LdsSlot {
     const double* sq;
     int s;
-    __device__ __forceinline__ double operator[](int k) const { return sq[k * kGradSlots + s]; }
+    __device__ __forceinline__ double operator[](int k) const { return sq[k * kSlots + s]; }
 };
 
 // One workgroup per gradient tile (a flux tile, or a consecutive sub-range of
@@ -349,9 +350,9 @@ struct LdsSlot {
 // staged once into LDS (structure of arrays, so that lanes reading different
 // slots of one plane spread over the banks), then every cell reads its vertex
 // and face neighbours from there.
-template <bool kSA>
+template <bool kSA, int kSlots>
 __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
-    __shared__ double sq[(kSA ? 9 : 8) * kGradSlots];
+    __shared__ double sq[(kSA ? 9 : 8) * kSlots];
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.gtile_off[t], nt = m.gtile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
@@ -374,10 +375,10 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
         for (int k = 0; k < 8; k += 2) {  // planes 0-7 = q, centroid: four 16-byte pair loads
             const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(k, f.ld, src));
-            sq[k * kGradSlots + sl] = v.x;
-            sq[(k + 1) * kGradSlots + sl] = v.y;
+            sq[k * kSlots + sl] = v.x;
+            sq[(k + 1) * kSlots + sl] = v.y;
         }
-        if constexpr (kSA) sq[8 * kGradSlots + sl] = sa.nu[src];
+        if constexpr (kSA) sq[8 * kSlots + sl] = sa.nu[src];
     }
     __syncthreads();
     if (!live) return;
@@ -391,8 +392,8 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
         const unsigned w = (k & 2) ? cur.y : cur.x;
         return static_cast<int>((w >> ((k & 1) * 16)) & 0xffffu);
     };
-    const auto nbn = [&](int, int k) { return LdsSlot{sq, take(k, curn, nxtn, chn, nchn)}; };
-    const auto nbf = [&](int, int k) { return LdsSlot{sq, take(k, curf, nxtf, chf, nchf)}; };
+    const auto nbn = [&](int, int k) { return LdsSlot<kSlots>{sq, take(k, curn, nxtn, chn, nchn)}; };
+    const auto nbf = [&](int, int k) { return LdsSlot<kSlots>{sq, take(k, curf, nxtf, chf, nchf)}; };
     // set-up-time face vectors, transposed per tile like the slot lists
     const long long vb = m.face_vec_base[t];
     const long long vw = (m.face_vec_base[t + 1] - vb) / (3LL * nt);
@@ -404,11 +405,12 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
     };
     if constexpr (kSA) {
         // the module's nu_tilde gradient in the same pass (k_sa_gradient's arithmetic)
-        const LdsScalar sc{sq + 8 * kGradSlots, static_cast<int>(threadIdx.x), sa.g, m.nc};
-        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, fvec,
-                              sc);
+        const LdsScalar sc{sq + 8 * kSlots, static_cast<int>(threadIdx.x), sa.g, m.nc};
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot<kSlots>{sq, static_cast<int>(threadIdx.x)}, nbn, nbf,
+                              fvec, sc);
     } else {
-        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot{sq, static_cast<int>(threadIdx.x)}, nbn, nbf, fvec);
+        gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot<kSlots>{sq, static_cast<int>(threadIdx.x)}, nbn, nbf,
+                              fvec);
     }
 }
 
@@ -928,11 +930,18 @@ bool launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
     if (m.nc <= 0) return false;
     if (m.tile_halo_off != nullptr) {
         const dim3 g(static_cast<unsigned>(m.ngtiles)), b(static_cast<unsigned>(m.gblock));
+        const bool big = m.gslots > kGradSlots;
         if (sa) {
-            hipLaunchKernelGGL(k_gradient_limiter_lds<true>, g, b, 0, s, m, f, nm, *sa);
+            if (big)
+                hipLaunchKernelGGL((k_gradient_limiter_lds<true, kGradSlotsBig>), g, b, 0, s, m, f, nm, *sa);
+            else
+                hipLaunchKernelGGL((k_gradient_limiter_lds<true, kGradSlots>), g, b, 0, s, m, f, nm, *sa);
             return true;
         }
-        hipLaunchKernelGGL(k_gradient_limiter_lds<false>, g, b, 0, s, m, f, nm, SaArgs{});
+        if (big)
+            hipLaunchKernelGGL((k_gradient_limiter_lds<false, kGradSlotsBig>), g, b, 0, s, m, f, nm, SaArgs{});
+        else
+            hipLaunchKernelGGL((k_gradient_limiter_lds<false, kGradSlots>), g, b, 0, s, m, f, nm, SaArgs{});
         return false;
     }
     hipLaunchKernelGGL(k_gradient_limiter, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm);
