@@ -192,6 +192,7 @@ def main():
             "ms_per_launch": round(flux_ms, 5),
         },
         "cpu_baseline": cpu,
+        "stream_ceiling": stream_ceiling(),
         "stage_ms": {"primitives_ghosts": round(stage_ms[0], 4), "gradients_limiter": round(stage_ms[1], 4),
                      "ghost_gradients": round(stage_ms[2], 4), "flux": round(stage_ms[3], 4)},
         "flux_kernel_mcells_s": round(nc / (flux_ms * 1e-3) / 1e6, 1),
@@ -242,6 +243,47 @@ def sa_residual(a):
             "ms": round(ms, 4), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
             "stage_ms": [round(float(x), 4) for x in stages[:4]],
             "set_turbulence_s": round(t_set, 2)}
+
+
+def stream_ceiling(nbytes=1 << 31, reps=10):
+    """Measured HBM ceiling of this device, for context beside the 8 TB/s
+    peak: the HIP runtime's device-to-device copy of nbytes (read + write
+    bytes counted) and its fill of nbytes (write bytes), timed with HIP
+    events over `reps` calls after warm-up."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        vp = C.c_void_p
+        src, dst = vp(), vp()
+        if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) != 0 or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) != 0:
+            return {"error": "hipMalloc"}
+        e0, e1 = vp(), vp()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        out = {}
+        ops = (("copy", lambda: hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None), 2 * nbytes),
+               ("fill", lambda: hip.hipMemsetAsync(dst, 0, C.c_size_t(nbytes), None), nbytes))
+        for name, fn, moved in ops:
+            for _ in range(3):
+                fn()
+            hip.hipDeviceSynchronize()
+            hip.hipEventRecord(e0, None)
+            for _ in range(reps):
+                fn()
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+            out[f"{name}_GBps"] = round(moved / (ms.value / reps * 1e-3) / 1e9, 1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        out["bytes"] = nbytes
+        out["note"] = "HIP runtime device-to-device copy (read + write bytes) and fill (write bytes) of 2 GiB"
+        return out
+    except OSError as e:  # the ceiling is context, not part of the measured path
+        return {"error": str(e)}
 
 
 def time_to_drop(a):
