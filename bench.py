@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--drop-previous-update", action="store_true",
                     help="also time the previous-update guess alone (the network with all parameters zero)")
     ap.add_argument("--drop-max-steps", type=int, default=12000)
+    ap.add_argument("--tet-sa", default="60",
+                    help="hexahedra per direction of the tetrahedral SA bump residual (each split into 6 tets), or off")
     ap.add_argument("--sa", choices=["on", "off"], default="on",
                     help="also time the residual with the Spalart-Allmaras module on the same mesh")
     ap.add_argument("--pmc-run", action="store_true",
@@ -203,9 +205,49 @@ def main():
     solver.close()
     if a.sa != "off":
         out["sa_residual"] = sa_residual(a)
+    if a.tet_sa != "off":
+        out["tet_sa_residual"] = tet_sa_residual(a, int(a.tet_sa))
     if a.drop_size != "off":
         out["time_to_drop"] = time_to_drop(a)
     print(json.dumps(out))
+
+
+def tet_sa_residual(a, n):
+    """Residual evaluation of the transonic tetrahedral bump with the
+    Spalart-Allmaras module (Mach 0.8, Re 1e6; n^3 hexahedra split into 6
+    Kuhn tetrahedra each, 5 % node jitter): the wide LDS gradient form, the
+    tiled flux and module kernels on 70-neighbour stencils."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    case = cases.bump_sa(ni=n, nj=n, nk=n)
+    t0 = time.perf_counter()
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s.init_uniform(case.rho, case.velocity, case.p)
+    U = s.get_state()
+    s.set_state(U * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal(U.shape)))
+    s.set_turbulence(cases.turbulence(case))
+    t_set = time.perf_counter() - t0
+    for _ in range(a.warmup):
+        s.eval_residual()
+    t = []
+    for _ in range(max(a.steps // 5, 5)):
+        t1 = time.perf_counter()
+        s.eval_residual()
+        t.append(time.perf_counter() - t1)
+    ms = 1e3 * float(np.median(t))
+    stages = s.stage_times()
+    flux_ms = s.time_flux_kernel(10)
+    nc = mesh.n_cells
+    s.close()
+    return {"workload": f"transonic bump, {n}x{n}x{n} hexahedra x 6 tetrahedra ({nc} cells), Mach 0.8, Re 1e6, "
+                        "viscous + Spalart-Allmaras (point-implicit module), node LSQ + Venkatakrishnan MUSCL",
+            "ms": round(ms, 4), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
+            "stage_ms": [round(float(x), 4) for x in stages[:4]], "flux_kernel_ms": round(flux_ms, 4),
+            "setup_s": round(t_set, 2)}
 
 
 def sa_residual(a):
