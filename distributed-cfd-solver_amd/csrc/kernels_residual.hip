@@ -335,13 +335,15 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
         });
 }
 
-// View of one LDS slot of the staged gradient kernel: plane k of slot s at
-// sq[k * kSlots + s].
+// View of one LDS slot of the staged gradient kernel: planes 0-7 (q,
+// centroid) stored as interleaved pairs like the cell planes, quantity k of
+// slot s at sq[((k / 2) kSlots + s) 2 + k % 2], so one 16-byte LDS read
+// brings two quantities of a neighbour.
 template <int kSlots>
 struct LdsSlot {
     const double* sq;
     int s;
-    __device__ __forceinline__ double operator[](int k) const { return sq[k * kSlots + s]; }
+    __device__ __forceinline__ double operator[](int k) const { return sq[((k >> 1) * kSlots + s) * 2 + (k & 1)]; }
 };
 
 // One workgroup per gradient tile (a flux tile, or a consecutive sub-range of
@@ -352,7 +354,7 @@ struct LdsSlot {
 // and face neighbours from there.
 template <bool kSA, int kSlots>
 __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m, FieldArgs f, NumArgs nm, SaArgs sa) {
-    __shared__ double sq[(kSA ? 9 : 8) * kSlots];
+    __shared__ __attribute__((aligned(16))) double sq[(kSA ? 9 : 8) * kSlots];  // 4 plane pairs (+ nu_tilde)
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.gtile_off[t], nt = m.gtile_off[t + 1] - c0;
     const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
@@ -375,8 +377,7 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
         const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
         for (int k = 0; k < 8; k += 2) {  // planes 0-7 = q, centroid: four 16-byte pair loads
             const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(k, f.ld, src));
-            sq[k * kSlots + sl] = v.x;
-            sq[(k + 1) * kSlots + sl] = v.y;
+            *reinterpret_cast<double2*>(sq + ((k >> 1) * kSlots + sl) * 2) = v;
         }
         if constexpr (kSA) sq[8 * kSlots + sl] = sa.nu[src];
     }
