@@ -656,7 +656,6 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.cf_slot.upload(tf.cell_slot, s);
         c.tile_off.upload(tl.tile_off, s);
         c.tile_face_off.upload(tf.tile_face_off, s);
-        c.tile_faces.upload(tf.faces, s);
         // the flux kernel's copy of the face planes, in the order of the tiles'
         // face lists: normal, area, centroid, and the packed (owner, other) ids
         const long long nte = static_cast<long long>(tf.faces.size());

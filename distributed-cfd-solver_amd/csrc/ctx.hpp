@@ -91,7 +91,7 @@ struct cfdg_ctx {
     DBuf<double> fp, tf;          // face planes: mesh order (fld), tile face-list order (tld)
     DBuf<int2> fcells;            // (owner, neighbour or ghost) per face
     long long fld = 0, tld = 0;
-    DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off, tile_faces;
+    DBuf<int> cf_off, cf, cf_slot, ccf_off, ccf, ccn_off, ccn, bface_bc, tile_off, tile_face_off;
     DBuf<int> tile_halo_off, tile_halo, gtile_off;  // gradient tiles (the flux tiles or sub-ranges of them)
     int ngtiles = 0, gblock = 0, gslots = 0;
     DBuf<std::uint16_t> ccn_slot, ccf_slot;
@@ -202,7 +202,6 @@ struct cfdg_ctx {
         a.bface_bc = bface_bc.get();
         a.tile_off = tile_off.get();
         a.tile_face_off = tile_face_off.get();
-        a.tile_faces = tile_faces.get();
         a.tile_halo_off = tile_halo_off.get();
         a.gtile_off = gtile_off.get();
         a.ngtiles = ngtiles;

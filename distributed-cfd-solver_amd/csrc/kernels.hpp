@@ -116,7 +116,6 @@ struct MeshArgs {
     const int* bface_bc;      // boundary face -> patch id
     const int* tile_off;      // tiles + 1
     const int* tile_face_off; // tiles + 1
-    const int* tile_faces;
     // LDS-staged gradient stencils over the gradient tiles (null when they do not fit)
     const int* gtile_off;     // gradient tiles + 1: the flux tiles, or consecutive sub-ranges of them
     int ngtiles;
