@@ -263,15 +263,17 @@ void spmv_stored(cfdg_ctx& c, const T* x, T* y, const double* gate = nullptr) {
 // The backward sweep starts with the group the forward sweep ended with: those
 // rows read only neighbours that the forward sweep already left final, so their
 // backward update equals their forward one bit for bit and is skipped.
+// z_zeroed: the caller's preceding kernel already set z to 0.
 template <class T>
-void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr) {
+void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr,
+                   bool z_zeroed = false) {
     const long long n5 = 5LL * c.nc;
     if (P.kind == CFDG_PRECOND_NONE) {
         hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(T), hipMemcpyDeviceToDevice, c.stream),
                   "copy");
         return;
     }
-    hip_check(hipMemsetAsync(z, 0, static_cast<std::size_t>(n5) * sizeof(T), c.stream), "memset");
+    if (!z_zeroed) hip_check(hipMemsetAsync(z, 0, static_cast<std::size_t>(n5) * sizeof(T), c.stream), "memset");
     const int gf = (P.order == CFDG_SGS_SEQUENTIAL) ? 2 : 0;
     const auto b = c.bargs();
     for (int sweep = 0; sweep < 2; ++sweep) {
@@ -353,8 +355,9 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
             for (int b = 0; b < batch && enq <= lp.max_iter; ++b) {
                 ++enq;
                 cfdg::launch_bicg_begin(ks, c.stream);
-                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, c.stream);
-                precond_apply<T>(c, P, w.p, w.ph, ks);
+                const bool sgs = P.kind == CFDG_PRECOND_SGS;
+                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, sgs ? w.ph : nullptr, c.stream);
+                precond_apply<T>(c, P, w.p, w.ph, ks, sgs);
                 spmv_stored<T>(c, w.ph, w.v, ks);
                 {
                     cfdg::DotPairsT<T> pr{};
@@ -363,8 +366,8 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                     cfdg::launch_dots(N, 1, pr, c.partial.get(), c.dots.get(), c.stream, ks);
                 }
                 cfdg::launch_bicg_alpha(ks, c.dots.get(), c.stream);
-                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, c.stream);
-                precond_apply<T>(c, P, w.s, w.sh, ks);
+                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, sgs ? w.sh : nullptr, c.stream);
+                precond_apply<T>(c, P, w.s, w.sh, ks, sgs);
                 spmv_stored<T>(c, w.sh, w.t, ks);
                 {
                     cfdg::DotPairsT<T> pr{};

@@ -367,10 +367,12 @@ __global__ void k_bicg_post(double* ks, const double* rr) {
 
 // p = r on the first iteration, else p = ((p beta) + (-beta omega) v) + r
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p) {
+__global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p,
+                                                       T* zero) {
     if (ks[kKsStatus] != 0.0) return;
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (zero) zero[i] = static_cast<T>(0);  // the preconditioner's output, which its sweep starts from
     if (ks[kKsFirst] != 0.0) {
         p[i] = r[i];
         return;
@@ -383,10 +385,12 @@ __global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double
 
 // s = r + (-alpha) v
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv) {
+__global__ void __launch_bounds__(kBlock) k_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv,
+                                                         T* zero) {
     if (ks[kKsStatus] != 0.0) return;
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (zero) zero[i] = static_cast<T>(0);
     sv[i] = r[i] + static_cast<T>(-ks[kKsAlpha]) * v[i];
 }
 
@@ -657,19 +661,22 @@ void launch_bicg_post(double* ks, const double* rr, hipStream_t s) {
 }
 
 template <class T>
-void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_update_p_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, p);
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, T* zero, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_update_p_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, p, zero);
 }
-template void launch_update_p_g<double>(long long, const double*, const double*, const double*, double*, hipStream_t);
-template void launch_update_p_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
+template void launch_update_p_g<double>(long long, const double*, const double*, const double*, double*, double*,
+                                        hipStream_t);
+template void launch_update_p_g<float>(long long, const double*, const float*, const float*, float*, float*,
+                                       hipStream_t);
 
 template <class T>
-void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_axpy_alpha_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, sv);
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, T* zero, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy_alpha_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, sv, zero);
 }
-template void launch_axpy_alpha_g<double>(long long, const double*, const double*, const double*, double*,
+template void launch_axpy_alpha_g<double>(long long, const double*, const double*, const double*, double*, double*,
                                           hipStream_t);
-template void launch_axpy_alpha_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
+template void launch_axpy_alpha_g<float>(long long, const double*, const float*, const float*, float*, float*,
+                                         hipStream_t);
 
 template <class T>
 void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,

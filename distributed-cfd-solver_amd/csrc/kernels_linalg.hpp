@@ -72,9 +72,10 @@ void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s);
 void launch_bicg_omega(double* ks, const double* dots, hipStream_t s);
 void launch_bicg_post(double* ks, const double* rr, hipStream_t s);
 template <class T>
-void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s);
+// zero (may be null) is set to 0 alongside: the next preconditioner output
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, T* zero, hipStream_t s);
 template <class T>
-void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s);
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, T* zero, hipStream_t s);
 template <class T>
 void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,
                         hipStream_t s);
