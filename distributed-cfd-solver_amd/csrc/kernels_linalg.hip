@@ -117,7 +117,7 @@ template <class TS, class TV>
 __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, const int* cols, const TS* values,
                                                  const TV* x, TV* y, const double* gate) {
     if (gate && gate[kKsStatus] != 0.0) return;
-    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     const long long row = t / 5;
     const int ii = static_cast<int>(t - row * 5);
     if (row >= n) return;
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int rw = lane / 5;  // row of the wave, 12 = idle lanes
     const int ii = lane - rw * 5;
-    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const long long wave = (static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) >> 6;
     const long long slot = wave * kSgsRowsPerWave + rw;
     const bool has_row = rw < kSgsRowsPerWave && slot < count;  // uniform over the row's five lanes
     const int i = has_row ? rows[slot] : 0;
