@@ -160,7 +160,20 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
     const bool has_row = rw < kSgsRowsPerWave && slot < count;  // uniform over the row's five lanes
     const int i = has_row ? rows[slot] : 0;
     TV acc = 0;
+    // FP32 factors: the row's LU factors and pivots are requested before the
+    // off-diagonal sweep, so their latency overlaps it instead of following the
+    // exchange.  FP64 factors are loaded after it (50 more live registers cost
+    // the FP64 kernel more occupancy than the overlap gains).
+    constexpr bool kEarlyLu = sizeof(TL) == sizeof(float);
+    TV Lr[25];
+    int pv[5];
+    const TL* L = lu + static_cast<long long>(i) * 25;
+    const std::uint8_t* p = piv + static_cast<long long>(i) * 5;
     if (has_row && ii < 5) {
+        if constexpr (kEarlyLu) {
+            for (int q = 0; q < 25; ++q) Lr[q] = load_as<TV>(L[q]);
+            for (int q = 0; q < 5; ++q) pv[q] = p[q];
+        }
         acc = r[static_cast<long long>(i) * 5 + ii];
         const int d = diag[i];
         const int kb = row_ptr[i], ke = row_ptr[i + 1];
@@ -182,10 +195,12 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
     TV s[5];
     for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, (base + q) & 63, 64);
     if (!has_row || ii >= 5) return;
-    const TL* L = lu + static_cast<long long>(i) * 25;
-    const std::uint8_t* p = piv + static_cast<long long>(i) * 5;
+    if constexpr (!kEarlyLu) {
+        for (int q = 0; q < 25; ++q) Lr[q] = load_as<TV>(L[q]);
+        for (int q = 0; q < 5; ++q) pv[q] = p[q];
+    }
     for (int k = 0; k < 5; ++k) {
-        const int pk = p[k];
+        const int pk = pv[k];
         if (pk != k) {
             TV tmp = s[k];
             TV sp = s[0];
@@ -196,13 +211,13 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
     }
     for (int a = 1; a < 5; ++a) {
         TV acc2 = s[a];
-        for (int b = 0; b < a; ++b) acc2 -= load_as<TV>(L[a * 5 + b]) * s[b];
+        for (int b = 0; b < a; ++b) acc2 -= Lr[a * 5 + b] * s[b];
         s[a] = acc2;
     }
     for (int a = 4; a >= 0; --a) {
         TV acc2 = s[a];
-        for (int b = a + 1; b < 5; ++b) acc2 -= load_as<TV>(L[a * 5 + b]) * s[b];
-        s[a] = acc2 / load_as<TV>(L[a * 5 + a]);
+        for (int b = a + 1; b < 5; ++b) acc2 -= Lr[a * 5 + b] * s[b];
+        s[a] = acc2 / Lr[a * 5 + a];
     }
     TV out = s[0];
     for (int q = 1; q < 5; ++q) out = (q == ii) ? s[q] : out;
