@@ -46,6 +46,8 @@ def parse():
                     help="hexahedra per direction of the tetrahedral SA bump residual (each split into 6 tets), or off")
     ap.add_argument("--sa", choices=["on", "off"], default="on",
                     help="also time the residual with the Spalart-Allmaras module on the same mesh")
+    ap.add_argument("--drop-profile", type=int, default=0, metavar="N",
+                    help="profiling mode: N implicit steps of the time-to-drop workload in each form, nothing else")
     ap.add_argument("--pmc-run", action="store_true",
                     help="profiling mode: one 1 GiB device copy (counter calibration) and flux-kernel launches only")
     return ap.parse_args()
@@ -64,6 +66,9 @@ def main():
         print(f"bench.py: one GPU only (requested --gpus {a.gpus}); the path is not sharded across GPUs",
               file=sys.stderr)
         sys.exit(2)
+    if a.drop_profile > 0:
+        print(json.dumps(drop_profile(a)), flush=True)
+        return
 
     import numpy as np
 
@@ -326,6 +331,44 @@ def stream_ceiling(nbytes=1 << 31, reps=10):
         return out
     except OSError as e:  # the ceiling is context, not part of the measured path
         return {"error": str(e)}
+
+
+def drop_profile(a):
+    """N implicit steps of the time-to-drop workload (the same plate, numerics,
+    noise and precision forms as time_to_drop) in the plain FP64 form and the
+    two mixed forms, each after 20 untimed steps, and nothing else: the
+    command a kernel trace of that workload runs under."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    ni, nj, nk = (int(x) for x in a.drop_size.split("x"))
+    lp = cases.default_linparams()
+    forms = {"plain_fp64": None,
+             "fp32_krylov": cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1),
+             "bf16_blocks_fp32_krylov": cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=8,
+                                                            inner_rel_tol=0.1)}
+    out = {"workload": f"laminar plate {ni}x{nj}x{nk} hexahedra, unlimited MUSCL, CFL 1000, 2% seeded noise",
+           "steps": a.drop_profile}
+    for name, prec in forms.items():
+        case = cases.plate(ni=ni, nj=nj, nk=nk, limiter="UNLIMITED", cfl=1000.0)
+        s = cfdg.Solver(cfdg.Mesh.generate(**case.mesh), case.numerics, case.bcs)
+        s.init_uniform(case.rho, case.velocity, case.p)
+        U = s.get_state()
+        s.set_state(U * (1.0 + 0.02 * np.random.default_rng(1).standard_normal(U.shape)))
+        if prec is not None:
+            s.set_precision(prec)
+        for _ in range(20):
+            s.implicit_step(lp)
+        its = 0
+        t0 = time.perf_counter()
+        for _ in range(a.drop_profile):
+            its += s.implicit_step(lp).linear.iterations
+        dt = time.perf_counter() - t0
+        out[name] = {"ms_per_step": round(1e3 * dt / a.drop_profile, 3), "linear_iterations": its}
+        print(f"bench.py: drop profile {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
 
 
 def time_to_drop(a):
