@@ -151,6 +151,7 @@ def _load():
     sig = {
         "cfdg_mesh_generate": [vp, vp], "cfdg_mesh_from_view": [vp, vp], "cfdg_mesh_get_view": [vp, vp],
         "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp],
+        "cfdg_mesh_cell_gid": [vp, vp],
         "cfdg_create": [vp, vp, vp, _i32, vp], "cfdg_destroy": [vp], "cfdg_sizes": [vp, vp, vp, vp],
         "cfdg_set_state": [vp, vp, _i64], "cfdg_get_state": [vp, vp, _i64], "cfdg_init_uniform": [vp, _d, vp, _d],
         "cfdg_eval_residual": [vp], "cfdg_get_primitives": [vp, vp, _i64],
@@ -162,7 +163,8 @@ def _load():
         "cfdg_explicit_step": [vp, vp], "cfdg_set_unsteady": [vp, _d, _i32],
         "cfdg_run_unsteady": [vp, vp, _i64, _i64, _d, vp],
         "cfdg_write_solution_vtu": [vp, vp, C.c_char_p, C.c_char_p],
-        "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
+        "cfdg_stage_times": [vp, vp], "cfdg_sgs_apply": [vp, _i32, vp, vp],
+        "cfdg_sgs_row_groups": [vp, _i32, vp, vp, vp], "cfdg_spmv": [vp, vp, vp], "cfdg_time_flux_kernel": [vp, _i32, vp],
         "cfdg_set_precision": [vp, vp], "cfdg_get_precision": [vp, vp], "cfdg_run_steady": [vp, vp, vp, vp],
         "cfdg_boundary_integrals": [vp, vp, vp, vp, _i32],
         "cfdg_gnn_create": [C.c_uint64, vp], "cfdg_gnn_destroy": [vp], "cfdg_gnn_num_params": [vp],
@@ -173,6 +175,7 @@ def _load():
         "cfdg_gnn_get_sample": [vp, _i32, vp, vp, vp, vp, vp, vp, vp, vp], "cfdg_gnn_predict": [vp, vp, vp],
         "cfdg_set_warm_start": [vp, vp], "cfdg_warm_start_stats": [vp, vp, vp],
         "cfdg_set_turbulence": [vp, vp, vp], "cfdg_set_turbulence_state": [vp, vp],
+        "cfdg_set_eddy_viscosity": [vp, vp, _i32],
         "cfdg_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
@@ -287,6 +290,14 @@ class Mesh:
             lib.cfdg_mesh_patch_name(self._h, p, buf, 128)
             out.append(buf.value.decode())
         return out
+
+    def cell_gid(self):
+        """Index of each cell at generation / copy, carried through renumbering."""
+        g = np.empty(self.n_cells, np.int64)
+        rc = lib.cfdg_mesh_cell_gid(self._h, _ptr(g))
+        if rc != OK:
+            raise CfdgError(rc, "cell_gid")
+        return g
 
     def reorder_rcm(self):
         rc = lib.cfdg_mesh_reorder_rcm(self._h)
@@ -404,6 +415,14 @@ class Solver:
         self._chk(lib.cfdg_sgs_apply(self._h, order, _ptr(r), _ptr(z)))
         return z
 
+    def sgs_row_groups(self, order):
+        """(forward group, backward group) of every mesh row and the group counts."""
+        f = np.empty(self.n_cells, np.int32)
+        b = np.empty(self.n_cells, np.int32)
+        n = np.empty(2, np.int32)
+        self._chk(lib.cfdg_sgs_row_groups(self._h, order, _ptr(f), _ptr(b), _ptr(n)))
+        return f, b, (int(n[0]), int(n[1]))
+
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
@@ -475,6 +494,14 @@ class Solver:
         rho_nu = np.ascontiguousarray(rho_nu, dtype=np.float64)
         assert rho_nu.shape == (self.n_cells,)
         self._chk(lib.cfdg_set_turbulence_state(self._h, _ptr(rho_nu)))
+
+    def set_eddy_viscosity(self, mut):
+        """Prescribed mu_t of every cell and ghost (n_total, mesh order), or None to clear."""
+        if mut is None:
+            self._chk(lib.cfdg_set_eddy_viscosity(self._h, None, 0))
+            return
+        mut = np.ascontiguousarray(mut, dtype=np.float64)
+        self._chk(lib.cfdg_set_eddy_viscosity(self._h, _ptr(mut), mut.size))
 
     def turbulence(self):
         """Module fields of the last residual evaluation (mesh order): rho_nu, nu, mut, dist, res,

@@ -1170,6 +1170,29 @@ int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* zout
     });
 }
 
+int cfdg_sgs_row_groups(cfdg_ctx* c, int32_t sgs_order, int32_t* fwd_group, int32_t* bwd_group,
+                        int32_t* n_groups) {
+    return guarded(c, [&] {
+        if (sgs_order != CFDG_SGS_MULTICOLOUR && sgs_order != CFDG_SGS_SEQUENTIAL)
+            throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
+        if (sgs_order == CFDG_SGS_SEQUENTIAL) build_seq_groups(*c);
+        const int gf = (sgs_order == CFDG_SGS_SEQUENTIAL) ? 2 : 0;
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            const auto& off = c->h_gofs[gf + sweep];
+            int32_t* out = sweep == 0 ? fwd_group : bwd_group;
+            if (n_groups) n_groups[sweep] = static_cast<int32_t>(off.size()) - 1;
+            if (!out) continue;
+            // the rows the kernels are handed, read back from the device
+            std::vector<i32> rows(c->grows[gf + sweep].size());
+            c->d2h(rows.data(), c->grows[gf + sweep].get(), rows.size());
+            c->sync();
+            std::fill(out, out + c->nc, -1);
+            for (std::size_t g = 0; g + 1 < off.size(); ++g)
+                for (i32 k = off[g]; k < off[g + 1]; ++k) out[c->new2old[z(rows[z(k)])]] = static_cast<int32_t>(g);
+        }
+    });
+}
+
 int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
     return guarded(c, [&] {
         if (!x || !y) throw CfdgError(CFDG_ERR_ARG, "null vector");
@@ -1312,6 +1335,7 @@ int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulenc
             return;
         }
         if (t->model != CFDG_TURB_SA) throw CfdgError(CFDG_ERR_CONFIG, "unsupported turbulence model");
+        c->mut_ext = false;  // the module forms its own eddy viscosity
         if (t->update != CFDG_TURB_FROZEN && t->update != CFDG_TURB_POINT_IMPLICIT)
             throw CfdgError(CFDG_ERR_CONFIG, "turbulence update must be FROZEN or POINT_IMPLICIT");
         // config.cpp:175-186
@@ -1381,6 +1405,30 @@ int cfdg_set_turbulence_state(cfdg_ctx* c, const double* rho_nu) {
         c->sync();
         c->have_residual = false;
         c->have_dt = false;
+    });
+}
+
+int cfdg_set_eddy_viscosity(cfdg_ctx* c, const double* mut, int32_t n_total) {
+    return guarded(c, [&] {
+        c->have_residual = false;
+        c->have_dt = false;
+        if (!mut) {
+            c->mut_ext = false;
+            return;
+        }
+        if (c->sa_on()) throw CfdgError(CFDG_ERR_STATE, "the turbulence module forms its own eddy viscosity");
+        if (c->num.flow_model != CFDG_VISCOUS_FLOW)
+            throw CfdgError(CFDG_ERR_CONFIG, "eddy viscosity requires the viscous flow model");
+        if (n_total != c->ntot) throw CfdgError(CFDG_ERR_ARG, "eddy viscosity needs n_total values");
+        // cells to device order; ghosts keep their index n_cells + boundary face
+        std::vector<double> h(z(c->ntot));
+        for (int r = 0; r < c->nc; ++r) h[z(r)] = mut[c->new2old[z(r)]];
+        for (int g = c->nc; g < c->ntot; ++g) h[z(g)] = mut[g];
+        if (c->sa_mut.size() != z(c->ntot)) c->sa_mut.alloc(z(c->ntot));
+        if (c->sa_mdot_tile.size() != z(c->tld)) c->sa_mdot_tile.alloc(z(c->tld));
+        c->h2d(c->sa_mut.get(), h.data(), h.size());
+        c->sync();
+        c->mut_ext = true;
     });
 }
 

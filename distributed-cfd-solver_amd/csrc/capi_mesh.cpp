@@ -1,4 +1,5 @@
 // C ABI of the host-side mesh objects (include/cfdg.h, cfdg_mesh_*).
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -165,6 +166,12 @@ int cfdg_mesh_patch_name(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_
     if (!mesh || !buf || buflen <= 0) return CFDG_ERR_ARG;
     if (patch < 0 || patch >= static_cast<int32_t>(mesh->m.patch_names.size())) return CFDG_ERR_ARG;
     std::snprintf(buf, static_cast<std::size_t>(buflen), "%s", mesh->m.patch_names[static_cast<std::size_t>(patch)].c_str());
+    return CFDG_OK;
+}
+
+int cfdg_mesh_cell_gid(const cfdg_mesh* mesh, int64_t* gid) {
+    if (!mesh || !gid) return CFDG_ERR_ARG;
+    std::copy(mesh->m.cell_gid.begin(), mesh->m.cell_gid.end(), gid);
     return CFDG_OK;
 }
 

@@ -161,6 +161,10 @@ struct cfdg_ctx {
     DBuf<int> face_entry;                               // entry of each face in its owner tile's face list
     std::vector<double> h_sa_dist;                      // wall distance, mesh order
     bool sa_on() const { return turb.model == CFDG_TURB_SA; }
+    // prescribed eddy viscosity (cfdg_set_eddy_viscosity): sa_mut holds it and
+    // sa_mdot_tile takes the flux kernel's mass flux; the module stays off
+    bool mut_ext = false;
+    bool has_mut() const { return sa_on() || mut_ext; }
     cfdg::SaArgs sargs() const {
         cfdg::SaArgs a{};
         a.nu_inf = sa_nu_inf;
@@ -224,9 +228,9 @@ struct cfdg_ctx {
         f.res = res.get();
         f.lam = lam.get();
         f.bflux = bflux.get();
-        f.mut = sa_on() ? sa_mut.get() : nullptr;
+        f.mut = has_mut() ? sa_mut.get() : nullptr;
         f.mdot = sa_on() ? sa_mdot.get() : nullptr;
-        f.mdot_tile = sa_on() ? sa_mdot_tile.get() : nullptr;
+        f.mdot_tile = has_mut() ? sa_mdot_tile.get() : nullptr;
         return f;
     }
     cfdg::BsrArgs bargs() const {

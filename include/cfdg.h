@@ -152,6 +152,11 @@ int cfdg_mesh_generate(const cfdg_mesh_spec* spec, cfdg_mesh** out);
 int cfdg_mesh_from_view(const cfdg_mesh_view* view, cfdg_mesh** out);
 int cfdg_mesh_get_view(const cfdg_mesh* mesh, cfdg_mesh_view* view);
 int cfdg_mesh_patch_name(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_t buflen);
+/* Global id of each cell (n_cells values; MeshPart::cell_gid,
+   localmesh.hpp:17-102): the cell's index at generation or copy, carried
+   through renumbering, so gid[new] = old index after cfdg_mesh_reorder_rcm
+   (reorder.cpp:306-576 permutes cell_gid the same way). */
+int cfdg_mesh_cell_gid(const cfdg_mesh* mesh, int64_t* gid);
 /* Renumbers cells in reverse Cuthill-McKee order and re-sorts faces
    (src/mesh/reorder.cpp:135-188, 306-576). */
 int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh);
@@ -241,6 +246,17 @@ int cfdg_linear_solve(cfdg_ctx* ctx, const cfdg_linparams* lp, const double* b, 
    forward and a backward sweep (SgsPreconditioner::apply, preconditioners.cpp:
    171-194; LU of the diagonal blocks as in :62-93).  AoS vectors, 5 n_cells. */
 int cfdg_sgs_apply(cfdg_ctx* ctx, int32_t sgs_order, const double* r, double* z);
+/* Row groups of the device SGS sweeps (no reference counterpart; they
+   define the order that replaces the row loop of bsr_sgs_sweep,
+   bsr_helpers.hpp:390-435): fwd_group[i] / bwd_group[i] = the group in which
+   the forward / backward sweep updates mesh row i (groups run in ascending
+   order, rows of one group at once); n_groups[0..1] = the group counts.
+   MULTICOLOUR: forward groups are the colours, backward groups the colours
+   in reverse (the backward sweep skips its group 0, which the forward sweep
+   has just computed from the same inputs); SEQUENTIAL: the dependency levels
+   of the natural forward / backward row order.  Any pointer may be NULL. */
+int cfdg_sgs_row_groups(cfdg_ctx* ctx, int32_t sgs_order, int32_t* fwd_group, int32_t* bwd_group,
+                        int32_t* n_groups);
 /* y = A x with the current BSR values (BsrMatrix::apply, bsr_matrix.cpp:10-20). */
 int cfdg_spmv(cfdg_ctx* ctx, const double* x, double* y);
 
@@ -397,6 +413,17 @@ typedef struct {
 int cfdg_set_turbulence(cfdg_ctx* ctx, const cfdg_mesh* mesh, const cfdg_turbulence* turb);
 /* rho nu_tilde of every cell (mesh order) */
 int cfdg_set_turbulence_state(cfdg_ctx* ctx, const double* rho_nu);
+/* Prescribed eddy viscosity mu_t of every cell and boundary ghost (n_total
+   values, mesh order, ghost of boundary face f at n_cells + f - n_inner), for
+   a host-side turbulence model: the mut argument of ResidualKernel::apply
+   (residual_kernel.hpp:64-91; the viscous face flux reads both sides, ghosts
+   included, :153-154, 231-232, 284-285) and of JacobianKernel::apply
+   (jacobian_kernel.hpp:74-90; zero on the ghost side, :252-254), i.e. what
+   Solver::mut_ptr() hands them (solver.hpp:230, 253, 942-945).  The field
+   stays until changed; mut == NULL clears it.  Needs the viscous flow model;
+   CFDG_ERR_STATE while the turbulence module (which forms its own mu_t) is
+   on; cfdg_set_turbulence with a model clears it. */
+int cfdg_set_eddy_viscosity(cfdg_ctx* ctx, const double* mut, int32_t n_total);
 /* Fields of the module after the last residual evaluation, mesh order, each
    argument optional: rho_nu, nu, mut, dist, res (n_cells), grad
    (3 x n_cells), mdot (n_faces, face mass flux), diag (point-implicit

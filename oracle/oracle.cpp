@@ -422,6 +422,14 @@ struct oracle_ctx {
         std::vector<double> dist, rnu, nu, rho, mut, g, res, mdot, diag;
     } sa;
     bool sa_on() const { return sa.cfg.model == 1; }
+    // prescribed eddy viscosity (cells + ghosts), the mut argument of
+    // ResidualKernel::apply / JacobianKernel::apply; empty when not set
+    std::vector<double> mut_ext;
+    // eddy viscosity of cell or ghost i as the mean-flow flux sees it
+    double mut_at(size_t i) const {
+        if (sa_on()) return sa.mut[i];
+        return mut_ext.empty() ? 0.0 : mut_ext[i];
+    }
     // dual time stepping (solver.hpp:355-380): BDF order 0 = steady, else the
     // physical time step and the stored time levels U^n, U^{n-1} (5 planes of nc)
     int bdf_order = 0, bdf_cur = 1;
@@ -1063,8 +1071,8 @@ void compute_fluxes(oracle_ctx& c) {
             double Fv[5] = {0, 0, 0, 0, 0};
             double lv = 0.0;
             // eddy viscosity of the two sides, zero in BC ghosts (residual_kernel.hpp:229-230, 282-283)
-            const double mutL = c.sa_on() ? c.sa.mut[c0] : 0.0;
-            const double mutR = c.sa_on() ? c.sa.mut[c1] : 0.0;
+            const double mutL = c.mut_at(c0);
+            const double mutR = c.mut_at(c1);
             viscous_flux(c.eos, c.dinv[fs], c.dvx[fs], c.dvy[fs], c.dvz[fs], c0, c1, vv, mutL, mutR, nxf, nyf, nzf, Af, Fv, lv,
                          c.num.prandtl, c.num.prandtl_turb);
             for (int v = 1; v < 5; ++v) F[v] += Fv[v];
@@ -1495,8 +1503,8 @@ void assemble_jacobian(oracle_ctx& c) {
         if (visc) {
             double vL[25], vR[25];
             // jacobian_kernel.hpp:156-157, 209-210, 252-254: ghost side zero
-            const double mutL = c.sa_on() ? c.sa.mut[c0] : 0.0;
-            const double mutR = (c.sa_on() && inner) ? c.sa.mut[c1] : 0.0;
+            const double mutL = c.mut_at(c0);
+            const double mutR = inner ? c.mut_at(c1) : 0.0;
             viscous_jacobian(c.eos, c.dinv[fs], qL, qR, mutL, mutR, c.area[fs], vL, vR, c.num.prandtl, c.num.prandtl_turb);
             for (int i = 0; i < 25; ++i) {
                 dFL[i] += vL[i];
@@ -2219,8 +2227,33 @@ int oracle_wall_distance(oracle_ctx* c, int32_t stencil, int32_t max_sweeps, dou
     return 0;
 }
 
+// Prescribed eddy viscosity of every cell and ghost (n_total values), read by
+// the mean viscous flux (residual_kernel.hpp:153-154, 231-232, 284-285) and the
+// Jacobian (jacobian_kernel.hpp:156-157, 209-210, 252-254: ghost side zero);
+// mut == NULL clears it.
+int oracle_set_eddy_viscosity(oracle_ctx* c, const double* mut, int32_t n_total) {
+    if (!c) return -1;
+    if (!mut) {
+        c->mut_ext.clear();
+        c->have_residual = false;
+        return 0;
+    }
+    if (c->sa_on()) {
+        c->err = "the turbulence module forms its own eddy viscosity";
+        return -3;
+    }
+    if (n_total != c->ntot || c->num.flow_model != 1) {
+        c->err = "eddy viscosity needs n_total values and the viscous flow model";
+        return -1;
+    }
+    c->mut_ext.assign(mut, mut + n_total);
+    c->have_residual = false;
+    return 0;
+}
+
 int oracle_set_turbulence(oracle_ctx* c, const oracle_turbulence* t) {
     if (!c || !t) return -1;
+    if (t->model != 0) c->mut_ext.clear();
     if (t->model != 0 && c->bdf_order > 0) {
         c->err = "the turbulence module is not supported in dual time stepping";
         return -1;

@@ -132,6 +132,10 @@ int oracle_wall_distance(oracle_ctx* ctx, int32_t stencil, int32_t max_sweeps, d
    nu_inf), state, and the per-evaluation fields of the last residual (n_cells
    each; grad 3 x n_cells; mdot n_faces; diag the point-implicit diagonal). */
 int oracle_set_turbulence(oracle_ctx* ctx, const oracle_turbulence* t);
+/* Prescribed eddy viscosity mu_t of every cell and boundary ghost (n_total,
+   ghosts after the cells) for the mean viscous flux and Jacobian; NULL clears
+   it.  Not with the turbulence module. */
+int oracle_set_eddy_viscosity(oracle_ctx* ctx, const double* mut, int32_t n_total);
 int oracle_set_turbulence_state(oracle_ctx* ctx, const double* rho_nu);
 int oracle_get_turbulence(oracle_ctx* ctx, double* rho_nu, double* nu, double* mut, double* dist, double* res,
                           double* grad, double* mdot, double* diag);

@@ -39,6 +39,7 @@ def _load():
         "oracle_spmv": [vp, vp, vp], "oracle_boundary_integrals": [vp, vp, vp, vp, i32],
         "oracle_wall_distance": [vp, i32, i32, d, vp],
         "oracle_set_turbulence": [vp, vp], "oracle_set_turbulence_state": [vp, vp],
+        "oracle_set_eddy_viscosity": [vp, vp, i32],
         "oracle_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "oracle_sa_point": [d, d, d, d, d, d, vp],
         "oracle_explicit_step": [vp, vp], "oracle_set_unsteady": [vp, d, i32],
@@ -233,6 +234,14 @@ class Oracle:
     def set_turbulence_state(self, rho_nu):
         rho_nu = np.ascontiguousarray(rho_nu, dtype=np.float64)
         self._chk(lib.oracle_set_turbulence_state(self._h, _ptr(rho_nu)))
+
+    def set_eddy_viscosity(self, mut):
+        """Prescribed mu_t of every cell and ghost (n_total), or None to clear."""
+        if mut is None:
+            self._chk(lib.oracle_set_eddy_viscosity(self._h, None, 0))
+            return
+        mut = np.ascontiguousarray(mut, dtype=np.float64)
+        self._chk(lib.oracle_set_eddy_viscosity(self._h, _ptr(mut), mut.size))
 
     def turbulence(self):
         """Fields of the last residual evaluation: dict of rho_nu, nu, mut,

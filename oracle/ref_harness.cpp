@@ -277,10 +277,27 @@ void run_components(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryC
                                                 scalar<double>(r, "venkat_k"));
         }
     }
+    // an eddy-viscosity field given as input goes to the kernels' mut argument
+    // (residual_kernel.hpp:64-91, 96-169; jacobian_kernel.hpp:74-90)
+    const std::vector<double> mut = r.count("mut") ? get<double>(r, "mut") : std::vector<double>{};
+    if (!mut.empty() && mut.size() != nt) throw std::runtime_error("mut needs n_cells + boundary faces values");
+    const double* mutp = mut.empty() ? nullptr : mut.data();
     sc::ResidualKernel<EOS, sc::fluxes::HllcFlux, Recon, Phys> rk(mp, aux_conn, aux_geom, eos, phys);
-    rk.apply(qc, gxc, gyc, gzc, phc, rp, lam.data());
+    rk.apply(qc, gxc, gyc, gzc, phc, rp, lam.data(), mutp);
     std::vector<double> bnd_mass, bnd_energy;
-    rk.boundary_integrals(qc, gxc, gyc, gzc, phc, bnd_mass, bnd_energy);
+    rk.boundary_integrals(qc, gxc, gyc, gzc, phc, bnd_mass, bnd_energy, mutp);
+    if (r.count("jac_alpha")) {
+        // JacobianKernel::apply on these primitives, the given diagonal shift
+        // and the eddy-viscosity field, into the reference's own BSR pattern
+        const auto alpha = get<double>(r, "jac_alpha");
+        if (alpha.size() != nc) throw std::runtime_error("jac_alpha needs n_cells values");
+        linalg::SolverParams lpar;
+        solver::BSRLinearSystem<5> sys(mp, aux_conn, lpar, MPI_COMM_WORLD);
+        sc::JacobianKernel<EOS, sc::fluxes::HllcFlux, Phys> jk(mp, aux_conn, aux_geom, eos, phys);
+        jk.apply(qc, sys.matrix().rowPtr().data(), sys.matrix().cols().data(), sys.matrix().diagIndex().data(),
+                 sys.matrix().valuesData(), alpha.data(), mutp);
+        w.put("jac_values", sys.matrix().values());
+    }
 
     std::vector<double> qo, go, po, ro;
     for (int v = 0; v < 5; ++v) qo.insert(qo.end(), q[v].begin(), q[v].end());

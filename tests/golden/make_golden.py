@@ -9,9 +9,10 @@ diagonal shift, BSR pattern and Jacobian blocks, one BiCGSTAB+SGS solution,
 and the state after a few implicit steps from the freestream.  The time_*
 fixtures hold the state after forward Euler iterations and after a few
 physical steps of dual time stepping (BDF1, BDF2) from the uniform state;
-the vtu_* fixtures the reference's solution files after a few implicit steps.
+the vtu_* fixtures the reference's solution files after a few implicit steps;
+plate_mut the residual and Jacobian with a prescribed eddy-viscosity field.
 
-usage: python tests/golden/make_golden.py [time|vtu]
+usage: python tests/golden/make_golden.py [time|vtu|mut|rcm]
 """
 import os
 import sys
@@ -182,9 +183,58 @@ def make_vtu(name, case, steps, seed=11):
     print(f"{name}: {mesh.n_cells} cells, VTU {len(out['vtu_piece'])} bytes -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
-def make_rcm():
+def make_mut(seed=23):
+    """Viscous plate with a prescribed, non-uniform eddy-viscosity field (cells
+    and BC ghosts): the reference's ResidualKernel::apply and
+    boundary_integrals with that mut (residual_kernel.hpp:64-169) and its
+    JacobianKernel::apply with mut and a given diagonal shift
+    (jacobian_kernel.hpp:74-270).  The diagonal shift is an input: the
+    oracle's compute_dt from its own spectral radius, which this fixture
+    pins against the reference's."""
+    from oracle_py import Oracle
+    case = cases.plate(ni=8, nj=6, nk=4)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    arrays = mesh.arrays()
+    nc = mesh.n_cells
+    rho, (u, v, w), p = case.rho, case.velocity, case.p
+    T = p / (rho * cases.R_AIR)
+    E = rho * (cases.R_AIR / (cases.GAMMA - 1.0)) * T + 0.5 * rho * (u * u + v * v + w * w)
+    rng = np.random.default_rng(seed)
+    U0 = np.array([rho, rho * u, rho * v, rho * w, E])[:, None] * np.ones((5, nc))
+    U = U0 * (1.0 + 0.02 * rng.standard_normal(U0.shape))
+    ora = Oracle(mesh, case.numerics, case.bcs)
+    n_total = ora.n_total
+    # mu_t between 0 and 40 laminar viscosities, ghosts included (nonzero)
+    mut = cases.sutherland(T) * 40.0 * rng.random(n_total)
+    ora.set_state(U)
+    ora.set_eddy_viscosity(mut)
+    ora.eval_residual()
+    ora.compute_dt()
+    alpha = ora.dt()[1]
+    lp = case.linparams
+    rec = ref_run.case_records(arrays, mesh.n_patches, case.numerics, case.bcs, U, [rho, u, v, w, p], lp,
+                               steps=0, solver_path=0)
+    rec["mut"] = mut
+    rec["jac_alpha"] = np.ascontiguousarray(alpha)
+    out = ref_run.run(rec)
+    fx = {f"mesh_{k}": a for k, a in arrays.items()}
+    fx["n_patches"] = np.array([mesh.n_patches])
+    fx.update({f"num_{k}": a for k, a in numerics_dict(case.numerics).items()})
+    fx.update(bcs_dict(case.bcs))
+    fx["U"] = U
+    fx["init"] = np.array([rho, u, v, w, p])
+    fx["mut"] = mut
+    fx["jac_alpha"] = alpha
+    for k in ("q", "grad", "phi", "res", "lam", "bnd_mass", "bnd_energy", "jac_values"):
+        fx[f"ref_{k}"] = out[k]
+    path = os.path.join(HERE, "plate_mut.npz")
+    np.savez_compressed(path, **fx)
+    print(f"plate_mut: {nc} cells -> {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+def make_rcm(name="rcm_plate", case=None):
     """Reference renumbering of a natural-order mesh (reorder.cpp:135-188, 306-576)."""
-    case = cases.plate(ni=10, nj=6, nk=4, reorder=0)
+    case = case or cases.plate(ni=10, nj=6, nk=4, reorder=0)
     mesh = cfdg.Mesh.generate(**case.mesh)
     arrays = mesh.arrays()
     U = np.ones((5, mesh.n_cells))
@@ -194,8 +244,16 @@ def make_rcm():
     fx = {f"mesh_{k}": a for k, a in arrays.items()}
     for k in ("rcm_new2old", "rcm_face_owner", "rcm_face_neigh", "rcm_face_patch", "rcm_face_area", "rcm_cell_cx"):
         fx[k] = out[k]
-    np.savez_compressed(os.path.join(HERE, "rcm_plate.npz"), **fx)
-    print("rcm_plate written")
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **fx)
+    print(f"{name} written")
+
+
+def rcm_bump_case():
+    """Tetrahedral bump in natural order: the face graph has cells of degree
+    1 to 4, so the degree sort of the breadth-first search decides ties."""
+    c = cases.bump(ni=4, nj=4, nk=3)
+    c.mesh["reorder"] = 0
+    return c
 
 
 if __name__ == "__main__":
@@ -205,10 +263,14 @@ if __name__ == "__main__":
     if not only:
         for name, case, steps in configs():
             make(name, case, steps)
+    if only in ("", "rcm"):
         make_rcm()
+        make_rcm("rcm_bump", rcm_bump_case())
     if only in ("", "time"):
         for args in time_configs():
             make_time(*args)
+    if only in ("", "mut"):
+        make_mut()
     if only in ("", "vtu"):
         make_vtu("vtu_ramp", cases.ramp(ni=12, nj=4, nk=2), 3)
         make_vtu("vtu_bump", cases.bump(ni=3, nj=3, nk=3), 2)

@@ -102,6 +102,47 @@ def test_sgs_and_spmv_match_oracle(name):
     assert_close_scaled(yg, yo, np.abs(yo).max(), 1e-12, "SpMV")
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_sgs_row_groups_match_oracle(name):
+    """The rows each device sweep is handed, read back from the device: the
+    forward multicolour groups are the oracle's colours, the backward ones
+    the colours reversed; the sequential groups are the dependency levels of
+    the natural forward / backward sweeps.  Identical integers."""
+    case = CASES[name]
+    mesh, gpu, ora = _pair(case)
+    col, ncol, fl, nf, bl, nb = ora.row_groups()
+    f, b, n = gpu.sgs_row_groups(cfdg.SGS_MULTICOLOUR)
+    assert n == (ncol, ncol)
+    assert np.array_equal(f, col)
+    assert np.array_equal(b, ncol - 1 - col)
+    f, b, n = gpu.sgs_row_groups(cfdg.SGS_SEQUENTIAL)
+    assert n == (nf, nb)
+    assert np.array_equal(f, fl)
+    assert np.array_equal(b, bl)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_multicolour_sgs_matches_numpy_colour_order(name):
+    """Device multicolour SGS against an independent numpy block-SGS run in
+    colour order with no skipped group (tests/test_sgs_oracle.py pins that
+    numpy sweep to the reference-pinned sequential one): 1e-12 of max|z|."""
+    from test_sgs_oracle import numpy_sgs
+    case = CASES[name]
+    mesh, gpu, ora = _pair(case)
+    for s in (gpu, ora):
+        s.eval_residual()
+        s.compute_dt()
+        s.assemble_jacobian()
+    rp, cols, dg = ora.bsr_pattern()
+    B = ora.bsr_values()
+    col = ora.row_groups()[0]
+    rows = np.arange(mesh.n_cells)
+    r = np.random.default_rng(6).standard_normal(5 * mesh.n_cells)
+    zn = numpy_sgs(rp, cols, dg, B, r, rows[np.lexsort((rows, col))], rows[np.lexsort((rows, -col))])
+    zg = gpu.sgs_apply(cfdg.SGS_MULTICOLOUR, r)
+    assert_close_scaled(zg, zn, np.abs(zn).max(), 1e-12, "multicolour SGS vs numpy")
+
+
 @pytest.mark.parametrize("order", [cfdg.SGS_SEQUENTIAL, cfdg.SGS_MULTICOLOUR])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_linear_solve_matches_oracle(name, order):

@@ -157,3 +157,21 @@ def test_sa_steady_state_matches_oracle():
     fg = s.boundary_integrals()[2][walls].sum(axis=0)
     fo = o.boundary_integrals()[2][walls].sum(axis=0)
     assert np.linalg.norm(fg - fo) <= 1e-8 * np.linalg.norm(fo), (fg, fo)
+
+
+def test_boundary_zones_closed_form_on_device():
+    """The device's zone ghosts (wall -nu, inflow nu_inf, mirror elsewhere),
+    Green-Gauss nu_tilde gradient and boundary-face diffusion on the zone box
+    of tests/test_sa_oracle.py, against its closed form: 1e-12 of the maxima."""
+    from test_sa_oracle import zone_box, zone_box_closed_form
+    mesh, num, bcs, rho, T, p = zone_box()
+    s = cfdg.Solver(mesh, num, bcs)
+    s.init_uniform(rho, (0.0, 0.0, 0.0), p)
+    s.set_turbulence(cfdg.Turbulence.make(rho_inf=rho, p_inf=p))
+    nu = 2.5 * 3.0 * cases.sutherland(T) / rho
+    s.set_turbulence_state(np.full(mesh.n_cells, rho * nu))
+    s.eval_residual()
+    t = s.turbulence()
+    G, R, _ = zone_box_closed_form(mesh, rho, T, t["dist"])
+    assert np.abs(t["grad"] - G).max() <= 1e-12 * np.abs(G).max()
+    assert np.abs(t["res"] - R).max() <= 1e-12 * np.abs(R).max()

@@ -172,3 +172,80 @@ def test_point_implicit_update_converges_and_frozen_keeps_state():
     assert np.all(t["rho_nu"] >= 0.0)
     assert norms[-1] < 1e-2 * norms[0], (norms[0], norms[-1])
     assert t["mut"].max() > 0.0
+
+
+# ---- boundary zones (spalart_allmaras.hpp:50-55, zone_kind :452-468) --------
+# An orthogonal box at rest with a uniform nu_tilde = 2.5 nu_inf and one patch
+# of each zone: farfield (inflow: ghost nu_inf), subsonic and supersonic outlet
+# (outflow: mirror), adiabatic no-slip wall (wall: ghost -nu), slip wall and
+# symmetry (symmetry: mirror).  The mass flux is zero on every face, interior
+# faces carry no diffusion (uniform field, the skew correction cancels the
+# averaged gradient along d || n), so with the Green-Gauss face gradient:
+#   V grad_c = sum over c's boundary faces of (nu_g - nu) / 2 A n
+#   res_c    = - sum over c's boundary faces of coef (nu_g - nu) / |d| A
+#              + V (D - P - cb2 term)(grad_c),     coef = (mu + rho (nu + nu_g) / 2) / sigma,
+# |d| = 2 |x_f - x_c|, sources from sa_point (pinned above) at zero strain.
+
+ZONE_BOX_BCS = (("FARFIELD", "inflow"), ("SUBSONIC_OUTLET", "mirror"), ("NO_SLIP_WALL_HEAT_FLUX", "wall"),
+                ("SLIP_WALL", "mirror"), ("SYMMETRY", "mirror"), ("SUPERSONIC_OUTLET", "mirror"))
+
+
+def zone_box():
+    T, p = 300.0, 101325.0
+    rho = p / (cases.R_AIR * T)
+    mesh = cfdg.Mesh.generate(kind=0, ni=6, nj=5, nk=4, x0=0.0, x1=1.2, y0=0.0, y1=0.5, z0=0.0, z1=0.6)
+    num = cases.make_numerics(True, gradient="GREEN_GAUSS_FACE", limiter="UNLIMITED", cfl=10.0)
+    bcs = []
+    for kind, _ in ZONE_BOX_BCS:
+        kw = {"p": p, "t": T} if kind in ("FARFIELD", "SUBSONIC_OUTLET") else {}
+        bcs.append(cases.make_bc(kind, **kw))
+    return mesh, num, bcs, rho, T, p
+
+
+def zone_box_closed_form(mesh, rho, T, dist, wall_sign=-1.0):
+    """(gradient 3 x n, residual n) of the zone box; wall_sign = the wall
+    ghost's nu / the cell's (-1 for the model; +1 plants a mirror wall)."""
+    A = mesh.arrays()
+    nc, nf, ni = mesh.n_cells, mesh.n_faces, mesh.n_inner_faces
+    own, pat, area, vol = A["face_owner"], A["face_patch"], A["face_area"], A["cell_vol"]
+    n = np.stack([A["face_nx"], A["face_ny"], A["face_nz"]])
+    xf = np.stack([A["face_cx"], A["face_cy"], A["face_cz"]])
+    xc = np.stack([A["cell_cx"], A["cell_cy"], A["cell_cz"]])
+    mu = cases.sutherland(T)
+    nu_inf = 3.0 * mu / rho
+    nu = 2.5 * nu_inf
+    ghost = {"inflow": nu_inf, "mirror": nu, "wall": wall_sign * nu}
+    zone = [z for _, z in ZONE_BOX_BCS]
+    G = np.zeros((3, nc))
+    R = np.zeros(nc)
+    for f in range(ni, nf):
+        c, nu_g = own[f], ghost[zone[pat[f]]]
+        G[:, c] += 0.5 * (nu_g - nu) * area[f] * n[:, f]
+        d = 2.0 * np.linalg.norm(xf[:, f] - xc[:, c])
+        R[c] -= (mu + rho * 0.5 * (nu + nu_g)) / (2.0 / 3.0) * (nu_g - nu) / d * area[f]
+    G /= vol
+    for c in range(nc):
+        s = sa_point(rho, nu, mu, dist[c], 0.0, G[:, c] @ G[:, c])
+        R[c] += vol[c] * (s["D"] - s["P"] - s["cb2"])
+    return G, R, nu
+
+
+def test_boundary_zones_closed_form():
+    mesh, num, bcs, rho, T, p = zone_box()
+    o = Oracle(mesh, num, bcs)
+    o.init_uniform(rho, (0.0, 0.0, 0.0), p)
+    o.set_turbulence(cfdg.Turbulence.make(rho_inf=rho, p_inf=p))
+    nu = 2.5 * 3.0 * cases.sutherland(T) / rho
+    o.set_turbulence_state(np.full(mesh.n_cells, rho * nu))
+    o.eval_residual()
+    t = o.turbulence()
+    assert np.all(t["mdot"] == 0.0)
+    G, R, _ = zone_box_closed_form(mesh, rho, T, t["dist"])
+    assert np.abs(t["grad"] - G).max() <= 1e-12 * np.abs(G).max()
+    assert np.abs(t["res"] - R).max() <= 1e-12 * np.abs(R).max()
+    # every zone is touched, and the closed form tells a mirrored wall apart
+    own = mesh.array("face_owner")[mesh.n_inner_faces:]
+    assert len(set(mesh.array("face_patch")[mesh.n_inner_faces:])) == len(ZONE_BOX_BCS)
+    G2, R2, _ = zone_box_closed_form(mesh, rho, T, t["dist"], wall_sign=1.0)
+    wall_cells = own[mesh.array("face_patch")[mesh.n_inner_faces:] == 2]
+    assert np.abs(R2[wall_cells] - R[wall_cells]).min() > 1e-3 * np.abs(R).max()
