@@ -169,7 +169,10 @@ void eval_residual(cfdg_ctx& c) {
     cfdg::launch_primitives(m, f, c.nargs, c.stream);
     cfdg::launch_ghosts(m, f, c.nargs, c.bcs, c.stream);
     // module ghosts, then density / nu_tilde / eddy viscosity (solver.hpp:195-204)
-    if (c.sa_on()) cfdg::launch_sa_prepare(m, f, c.nargs, c.sargs(), c.stream);
+    if (c.sa_on()) {
+        cfdg::launch_sa_prepare(m, f, c.nargs, c.sargs(), c.stream);
+        c.sa_evaluated = true;
+    }
     hip_check(hipEventRecord(c.ev[1], c.stream), "event");
     const bool need_grad = c.nargs.muscl || c.nargs.viscous;
     if (need_grad) {
@@ -216,9 +219,16 @@ void assemble_jacobian(cfdg_ctx& c) {
     hip_check(hipEventRecord(c.ev[6], c.stream), "event");
     c.check_launch();
     c.sync();
+    c.have_jacobian = true;
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, c.ev[5], c.ev[6]), "elapsed");
     c.stage_ms[4] = ms;
+}
+
+void require_jacobian(const cfdg_ctx& c) {
+    if (!c.have_jacobian)
+        throw CfdgError(CFDG_ERR_STATE,
+                        "no block values in the current storage: cfdg_assemble_jacobian or cfdg_set_bsr_values first");
 }
 
 void fetch_dots(cfdg_ctx& c, int n) {
@@ -511,6 +521,7 @@ void linear_solve(cfdg_ctx& c, const cfdg_linparams& lp, const double* hb, const
     if (lp.sgs_order != CFDG_SGS_MULTICOLOUR && lp.sgs_order != CFDG_SGS_SEQUENTIAL)
         throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
     if (lp.max_iter < 0) throw CfdgError(CFDG_ERR_ARG, "max_iter must be >= 0");
+    require_jacobian(c);
     const std::size_t n5 = 5ULL * static_cast<std::size_t>(c.nc);
     if (hb) {
         const auto d = c.vec_to_device(hb);
@@ -914,6 +925,7 @@ void set_precision(cfdg_ctx& c, const cfdg_precision& p) {
         c.values.alloc(0);
         c.values.alloc(nnzb * block_stride(storage) * storage_bytes(storage));
         c.values.zero(c.stream);
+        c.have_jacobian = false;
     }
     const int lu_f32 = (p.krylov == CFDG_PREC_FP32) ? 1 : 0;
     c.lu.alloc(0);
@@ -1116,6 +1128,7 @@ int cfdg_set_bsr_values(cfdg_ctx* c, const double* values) {
         for (std::size_t k = 0; k < c->blk_dev.size(); ++k)
             std::memcpy(h.data() + z(c->blk_dev[k]) * 25, values + k * 25, 25 * sizeof(double));
         upload_values(*c, h);
+        c->have_jacobian = true;
     });
 }
 
@@ -1147,6 +1160,7 @@ int cfdg_sgs_apply(cfdg_ctx* c, int32_t sgs_order, const double* r, double* zout
         if (!r || !zout) throw CfdgError(CFDG_ERR_ARG, "null vector");
         if (sgs_order != CFDG_SGS_MULTICOLOUR && sgs_order != CFDG_SGS_SEQUENTIAL)
             throw CfdgError(CFDG_ERR_CONFIG, "unknown SGS order");
+        require_jacobian(*c);
         const std::size_t n5 = 5ULL * z(c->nc);
         Precond P;
         P.kind = CFDG_PRECOND_SGS;
@@ -1196,6 +1210,7 @@ int cfdg_sgs_row_groups(cfdg_ctx* c, int32_t sgs_order, int32_t* fwd_group, int3
 int cfdg_spmv(cfdg_ctx* c, const double* x, double* y) {
     return guarded(c, [&] {
         if (!x || !y) throw CfdgError(CFDG_ERR_ARG, "null vector");
+        require_jacobian(*c);
         const std::size_t n5 = 5ULL * z(c->nc);
         const auto d = c->vec_to_device(x);
         c->h2d(c->s.get(), d.data(), n5);
@@ -1336,6 +1351,7 @@ int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulenc
         }
         if (t->model != CFDG_TURB_SA) throw CfdgError(CFDG_ERR_CONFIG, "unsupported turbulence model");
         c->mut_ext = false;  // the module forms its own eddy viscosity
+        c->sa_evaluated = false;
         if (t->update != CFDG_TURB_FROZEN && t->update != CFDG_TURB_POINT_IMPLICIT)
             throw CfdgError(CFDG_ERR_CONFIG, "turbulence update must be FROZEN or POINT_IMPLICIT");
         // config.cpp:175-186

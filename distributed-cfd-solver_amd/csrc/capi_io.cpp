@@ -1,8 +1,9 @@
 // C ABI of the solution output (include/cfdg.h, cfdg_write_solution_vtu): the
 // solution files of one rank as the reference writes them -- a VTU piece with
-// the mesh, a rank array and the cell fields rho, u, v, w, pressure, mach, and
-// its PVTU master (Solver::write_fields, solver.hpp:856-898;
-// io::vtk::write_solution_vtu, vtu.cpp:20-81, 321-475).  The fields are formed
+// the mesh, a rank array and the cell fields rho, u, v, w, pressure, mach
+// (with the turbulence module also nu_tilde and mu_t, appended by the module,
+// spalart_allmaras.hpp:445-448), and its PVTU master (Solver::write_fields,
+// solver.hpp:856-898; io::vtk::write_solution_vtu, vtu.cpp:20-81, 321-475).  The fields are formed
 // on the device from the state and written in the mesh's cell order with the
 // reference's text formats, so the files are byte-identical for equal states.
 #include <hip/hip_runtime.h>
@@ -49,9 +50,10 @@ struct File {
     }
 };
 
-const char* const kFieldNames[6] = {"rho", "u", "v", "w", "pressure", "mach"};
+const char* const kFieldNames[8] = {"rho", "u", "v", "w", "pressure", "mach", "nu_tilde", "mu_t"};
 
-void write_piece(const std::string& path, const cfdg::Mesh& m, const std::vector<double>& fields) {
+// fields: nf planes of n_cells in mesh order, named kFieldNames[0 .. nf)
+void write_piece(const std::string& path, const cfdg::Mesh& m, const std::vector<double>& fields, int nf) {
     File out(path);
     FILE* f = out.f;
     const int nc = m.n_cells, np = m.n_nodes;
@@ -85,7 +87,7 @@ void write_piece(const std::string& path, const cfdg::Mesh& m, const std::vector
     std::fprintf(f, "        <DataArray type=\"Int32\" Name=\"rank\" format=\"ascii\">\n          ");
     for (int c = 0; c < nc; ++c) std::fprintf(f, "%d ", 0);
     std::fprintf(f, "\n        </DataArray>\n");
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < nf; ++i) {
         std::fprintf(f, "        <DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n          ", kFieldNames[i]);
         const double* v = fields.data() + z(i) * z(nc);
         for (int c = 0; c < nc; ++c) std::fprintf(f, "%.10e ", v[z(c)]);
@@ -97,7 +99,7 @@ void write_piece(const std::string& path, const cfdg::Mesh& m, const std::vector
     std::fprintf(f, "</VTKFile>\n");
 }
 
-void write_master(const std::string& path, const std::string& stem) {
+void write_master(const std::string& path, const std::string& stem, int nf) {
     File out(path);
     FILE* f = out.f;
     std::fprintf(f, "<?xml version=\"1.0\"?>\n");
@@ -105,7 +107,7 @@ void write_master(const std::string& path, const std::string& stem) {
     std::fprintf(f, "  <PUnstructuredGrid GhostLevel=\"0\">\n");
     std::fprintf(f, "    <PCellData>\n");
     std::fprintf(f, "      <PDataArray type=\"Int32\" Name=\"rank\"/>\n");
-    for (int i = 0; i < 6; ++i) std::fprintf(f, "      <PDataArray type=\"Float64\" Name=\"%s\"/>\n", kFieldNames[i]);
+    for (int i = 0; i < nf; ++i) std::fprintf(f, "      <PDataArray type=\"Float64\" Name=\"%s\"/>\n", kFieldNames[i]);
     std::fprintf(f, "    </PCellData>\n");
     std::fprintf(f, "    <PPoints>\n");
     std::fprintf(f, "      <PDataArray type=\"Float64\" NumberOfComponents=\"3\"/>\n");
@@ -128,16 +130,23 @@ extern "C" int cfdg_write_solution_vtu(cfdg_ctx* c, const cfdg_mesh* mesh, const
         d.alloc(6 * nc);
         cfdg::launch_output_fields(c->nc, c->nargs.gas, c->U.get(), d.get(), c->stream);
         c->check_launch();
-        std::vector<double> dev(6 * nc), fields(6 * nc);
+        // the module's nu_tilde and mu_t as of the last residual evaluation
+        // (pre_sweep, spalart_allmaras.hpp:263-281), zero before the first
+        const int nf = c->sa_on() ? 8 : 6;
+        std::vector<double> dev(z(nf) * nc, 0.0), fields(z(nf) * nc);
         c->d2h(dev.data(), d.get(), 6 * nc);
+        if (c->sa_on() && c->sa_evaluated) {
+            c->d2h(dev.data() + 6 * nc, c->sa_nu.get(), nc);
+            c->d2h(dev.data() + 7 * nc, c->sa_mut.get(), nc);
+        }
         c->sync();
-        for (std::size_t i = 0; i < 6; ++i)
+        for (std::size_t i = 0; i < z(nf); ++i)
             for (std::size_t cc = 0; cc < nc; ++cc) fields[i * nc + cc] = dev[i * nc + z(c->old2new[cc])];
         std::error_code ec;
         std::filesystem::create_directories(outdir, ec);
         const std::string base = std::string(outdir) + "/" + stem;
-        write_piece(base + "_00000.vtu", m, fields);
-        write_master(base + ".pvtu", stem);
+        write_piece(base + "_00000.vtu", m, fields, nf);
+        write_master(base + ".pvtu", stem, nf);
         return CFDG_OK;
     } catch (const CfdgError& e) {
         c->err = e.what();

@@ -103,6 +103,7 @@ struct cfdg_ctx {
     long long ld = 0;  // cell plane stride
     bool have_residual = false;
     bool have_dt = false;
+    bool have_jacobian = false;  // block values set (assembled, or given) in the current storage
     // BSR: mesh-order pattern for the ABI, device-order pattern for the kernels
     std::vector<std::int32_t> h_row_ptr, h_cols, h_diag;
     std::vector<long long> blk_dev;  // mesh block k -> device block
@@ -164,6 +165,7 @@ struct cfdg_ctx {
     // prescribed eddy viscosity (cfdg_set_eddy_viscosity): sa_mut holds it and
     // sa_mdot_tile takes the flux kernel's mass flux; the module stays off
     bool mut_ext = false;
+    bool sa_evaluated = false;  // sa_nu / sa_mut hold a residual evaluation's values
     bool has_mut() const { return sa_on() || mut_ext; }
     cfdg::SaArgs sargs() const {
         cfdg::SaArgs a{};

@@ -325,9 +325,13 @@ int cfdg_explicit_step(cfdg_ctx* ctx, cfdg_step_result* out);
  *                          one rank: <outdir>/<stem>_00000.vtu (the mesh, a
  *                          rank array and the cell fields rho, u, v, w,
  *                          pressure, mach of the current state, formed on the
- *                          device) and its master <outdir>/<stem>.pvtu, in the
- *                          reference's text formats; outdir is created if
- *                          needed.  mesh = the mesh the context was created on.
+ *                          device; with the turbulence module also nu_tilde and
+ *                          mu_t of the last residual evaluation, zero before
+ *                          the first, as the module appends them,
+ *                          spalart_allmaras.hpp:445-448) and its master
+ *                          <outdir>/<stem>.pvtu, in the reference's text
+ *                          formats; outdir is created if needed.  mesh = the
+ *                          mesh the context was created on.
  */
 int cfdg_write_solution_vtu(cfdg_ctx* ctx, const cfdg_mesh* mesh, const char* outdir, const char* stem);
 int cfdg_set_unsteady(cfdg_ctx* ctx, double dt, int32_t order);
@@ -360,8 +364,10 @@ typedef struct {
     double inner_rel_tol;   /* each inner solve stops at max(inner_rel_tol ||r||, 0.9 tol) (0 < inner_rel_tol < 1) */
 } cfdg_precision;
 
-/* Selects the form; reallocates the block storage (the next solve needs a new
-   cfdg_assemble_jacobian or cfdg_set_bsr_values).  cfdg_get_bsr_values /
+/* Selects the form.  When the block storage changes it is reallocated, and
+   cfdg_linear_solve, cfdg_sgs_apply and cfdg_spmv return CFDG_ERR_STATE until
+   cfdg_assemble_jacobian or cfdg_set_bsr_values fills it (as they do on a
+   new context); with the same storage the blocks are kept.  cfdg_get_bsr_values /
    cfdg_set_bsr_values convert between the storage and double; cfdg_spmv
    multiplies the stored blocks in FP64; cfdg_sgs_apply runs the form's SGS. */
 int cfdg_set_precision(cfdg_ctx* ctx, const cfdg_precision* prec);

@@ -533,8 +533,47 @@ void run_timing(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfi
     w.put("time_setup", &t_setup, 1);
 }
 
+// BSRLinearSystem<5>::solve (BiCGSTAB + the configured preconditioner,
+// bsr_linear_system.hpp:93-97) on given block values, right-hand side and
+// initial guess, in the reference's own pattern of the mesh.
+void run_linear_system(const Recs& r, const mesh::MeshPart& mp, Writer& w) {
+    mesh::MeshAuxConnectivity aux_conn;
+    aux_conn.add_connectivity(mp, mesh::AuxConnType::CellCellsByFace);
+    linalg::SolverParams lp;
+    lp.type = linalg::SolverType::BICGSTAB;
+    lp.precond_type =
+        scalar<std::int32_t>(r, "precond") == 1 ? linalg::PreconditionerType::SGS : linalg::PreconditionerType::None;
+    lp.relative_tolerance = scalar<double>(r, "rel_tol");
+    lp.absolute_tolerance = scalar<double>(r, "abs_tol");
+    lp.max_iterations = scalar<std::int32_t>(r, "max_iter");
+    solver::BSRLinearSystem<5> sys(mp, aux_conn, lp, MPI_COMM_WORLD);
+    const auto vals = get<double>(r, "lin_values");
+    const auto b = get<double>(r, "lin_b");
+    const auto x0 = get<double>(r, "lin_x0");
+    const std::size_t n5 = 5 * static_cast<std::size_t>(mp.n_cells);
+    if (vals.size() != sys.matrix().values().size() || b.size() != n5 || x0.size() != n5)
+        throw std::runtime_error("lin_values / lin_b / lin_x0 sizes");
+    std::copy(vals.begin(), vals.end(), sys.matrix().valuesData());
+    std::copy(b.begin(), b.end(), sys.rhs_data());
+    std::copy(x0.begin(), x0.end(), sys.du_data());
+    sys.solve();
+    w.put("lin_x", sys.du_data(), n5);
+    const auto& lr = sys.last_result();
+    const std::int32_t st[2] = {static_cast<std::int32_t>(lr.status), lr.iterations};
+    w.put("lin_status", st, 2);
+    const double nr[2] = {lr.initial_residual, lr.final_residual};
+    w.put("lin_norms", nr, 2);
+    w.put("row_ptr", sys.matrix().rowPtr());
+    w.put("cols", sys.matrix().cols());
+    w.put("diag", sys.matrix().diagIndex());
+}
+
 template <typename Phys, typename Recon>
 void run_all(const Recs& r, const mesh::MeshPart& mp, const bc::BoundaryConfig& bcfg, const Phys& phys, Writer& w) {
+    if (r.count("lin_values")) {
+        run_linear_system(r, mp, w);
+        return;
+    }
     if (r.count("time_reps") && scalar<std::int32_t>(r, "time_reps") > 0) {
         run_timing<Phys, Recon>(r, mp, bcfg, phys, w);
         return;

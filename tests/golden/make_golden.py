@@ -12,7 +12,7 @@ physical steps of dual time stepping (BDF1, BDF2) from the uniform state;
 the vtu_* fixtures the reference's solution files after a few implicit steps;
 plate_mut the residual and Jacobian with a prescribed eddy-viscosity field.
 
-usage: python tests/golden/make_golden.py [time|vtu|mut|rcm]
+usage: python tests/golden/make_golden.py [time|vtu|mut|rcm|linsys]
 """
 import os
 import sys
@@ -232,6 +232,45 @@ def make_mut(seed=23):
     print(f"plate_mut: {nc} cells -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+def make_linsys():
+    """The reference's BSRLinearSystem<5>::solve on identity blocks with
+    b = ones and a zero guess, on two plates: 8x8x5 (320 cells, 5 N = 1600 =
+    40^2) and 8x8x4 (256 cells, 5 N = 1280).  With 5 N a perfect square
+    ||b||^2 is formed exactly, alpha = 1, s = t = 0 and the reference stops
+    with Diverged at tt == 0 without updating x (bicgstab.cpp:137-140);
+    otherwise alpha rounds off 1 and the solve converges in one iteration."""
+    fx = {}
+    for tag, nk in (("sq", 5), ("nsq", 4)):
+        case = cases.plate(ni=8, nj=8, nk=nk)
+        mesh = cfdg.Mesh.generate(**case.mesh)
+        arrays = mesh.arrays()
+        nc = mesh.n_cells
+        for prec in (0, 1):
+            lp = case.linparams
+            lp.preconditioner = prec
+            lp.sgs_order = cfdg.SGS_SEQUENTIAL
+            lp.rel_tol, lp.abs_tol, lp.max_iter = 1e-6, 1e-15, 50
+            rec = ref_run.case_records(arrays, mesh.n_patches, case.numerics, case.bcs, np.ones((5, nc)),
+                                       [case.rho, *case.velocity, case.p], lp, steps=0, solver_path=0)
+            # identity blocks in the pattern (the oracle's, bit-exact to the reference's)
+            from oracle_py import Oracle
+            _, cols, dg = Oracle(mesh, case.numerics, case.bcs).bsr_pattern()
+            vals = np.zeros((len(cols), 5, 5))
+            vals[dg] = np.eye(5)
+            rec.update(lin_values=vals.ravel(), lin_b=np.ones(5 * nc), lin_x0=np.zeros(5 * nc))
+            out = ref_run.run(rec)
+            assert np.array_equal(out["diag"], dg)
+            key = f"{tag}_p{prec}"
+            fx[f"{key}_mesh_nk"] = np.array([nk])
+            fx[f"{key}_lin"] = np.array([prec, lp.sgs_order, lp.rel_tol, lp.abs_tol, lp.max_iter], dtype=np.float64)
+            fx[f"{key}_values"] = vals.ravel()
+            fx[f"{key}_b"] = np.ones(5 * nc)
+            for k in ("lin_x", "lin_status", "lin_norms"):
+                fx[f"{key}_ref_{k}"] = out[k]
+            print(f"linsys {key}: status {out['lin_status']}, max|x| {np.abs(out['lin_x']).max()}")
+    np.savez_compressed(os.path.join(HERE, "linsys_identity.npz"), **fx)
+
+
 def make_rcm(name="rcm_plate", case=None):
     """Reference renumbering of a natural-order mesh (reorder.cpp:135-188, 306-576)."""
     case = case or cases.plate(ni=10, nj=6, nk=4, reorder=0)
@@ -271,6 +310,8 @@ if __name__ == "__main__":
             make_time(*args)
     if only in ("", "mut"):
         make_mut()
+    if only in ("", "linsys"):
+        make_linsys()
     if only in ("", "vtu"):
         make_vtu("vtu_ramp", cases.ramp(ni=12, nj=4, nk=2), 3)
         make_vtu("vtu_bump", cases.bump(ni=3, nj=3, nk=3), 2)

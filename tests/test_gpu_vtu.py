@@ -63,3 +63,45 @@ def test_mesh_mismatch_rejected(tmp_path):
     s.mesh = mesh_of(other)
     with pytest.raises(RuntimeError):
         s.write_solution_vtu(tmp_path, "x")
+
+
+def test_turbulence_fields_appended(tmp_path):
+    """With the turbulence module the piece and the master list rho, u, v, w,
+    pressure, mach, nu_tilde, mu_t in that order (the module appends its two
+    fields, spalart_allmaras.hpp:445-448; the reference's SA module does not
+    compile, so the order is the one that header states); the module fields
+    are those of the last residual evaluation, printed with the format of the
+    mean fields."""
+    import cfdg.cases as cases
+    case = cases.plate_sa(ni=12, nj=8, nk=4)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s.init_uniform(case.rho, case.velocity, case.p)
+    s.set_turbulence(cases.turbulence(case))
+    s.write_solution_vtu(tmp_path, "before")
+    before = (tmp_path / "before_00000.vtu").read_text()
+    s.implicit_step(cases.default_linparams())
+    s.eval_residual()
+    t = s.turbulence()
+    s.write_solution_vtu(tmp_path, "sa")
+    text = (tmp_path / "sa_00000.vtu").read_text()
+    master = (tmp_path / "sa.pvtu").read_text()
+    names = ["rho", "u", "v", "w", "pressure", "mach", "nu_tilde", "mu_t"]
+    pos = [text.index(f'Name="{n}" format="ascii">') for n in names]
+    assert pos == sorted(pos)
+    mpos = [master.index(f'<PDataArray type="Float64" Name="{n}"/>') for n in names]
+    assert mpos == sorted(mpos)
+
+    def field(txt, name):
+        body = txt.split(f'Name="{name}" format="ascii">', 1)[1].split("</DataArray>", 1)[0]
+        return body.split()
+
+    assert field(text, "nu_tilde") == [f"{x:.10e}" for x in t["nu"]]
+    assert field(text, "mu_t") == [f"{x:.10e}" for x in t["mut"]]
+    assert set(field(before, "mu_t")) == {f"{0.0:.10e}"}
+    # without the module the files keep the six mean fields only
+    s2 = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s2.init_uniform(case.rho, case.velocity, case.p)
+    s2.write_solution_vtu(tmp_path, "lam")
+    assert "nu_tilde" not in (tmp_path / "lam_00000.vtu").read_text()
+    assert "mu_t" not in (tmp_path / "lam.pvtu").read_text()
