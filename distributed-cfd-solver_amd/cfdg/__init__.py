@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcfdg.so")
 
 # --- constants (include/cfdg.h) ---------------------------------------------
 OK = 0
-ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH = -1, -2, -3, -4, -5, -6
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH, ERR_CAPACITY = -1, -2, -3, -4, -5, -6, -7
+NOALLOC_MAX_CELLS = 1 << 30
 INVISCID_FLOW, VISCOUS_FLOW = 0, 1
 FIRST_ORDER, MUSCL = 0, 1
 LIMITER = {"VENKAT": 0, "BARTH": 1, "VAN_ALBADA": 2, "UNLIMITED": 3}
@@ -177,6 +178,7 @@ def _load():
         "cfdg_set_turbulence": [vp, vp, vp], "cfdg_set_turbulence_state": [vp, vp],
         "cfdg_set_eddy_viscosity": [vp, vp, _i32],
         "cfdg_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        "cfdg_residual_norms_noalloc": [vp, _i64, _i32, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

@@ -1539,4 +1539,13 @@ int cfdg_time_flux_kernel(cfdg_ctx* c, int32_t reps, double* ms_per_launch) {
     });
 }
 
+int cfdg_residual_norms_noalloc(const double* res, int64_t ld, int32_t n_cells, double* l2, void* stream) {
+    if (!res || !l2 || n_cells < 0) return CFDG_ERR_ARG;
+    if (n_cells > CFDG_NOALLOC_MAX_CELLS) return CFDG_ERR_CAPACITY;
+    if (ld < n_cells || ld > static_cast<int64_t>(INT32_MAX)) return CFDG_ERR_ARG;
+    const hipError_t e = cfdg::residual_norms_noalloc(n_cells, static_cast<int>(ld), res, l2,
+                                                      static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? CFDG_OK : CFDG_ERR_HIP;
+}
+
 }  // extern "C"

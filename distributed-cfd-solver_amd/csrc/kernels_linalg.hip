@@ -19,10 +19,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 
 #include "kernels.hpp"
 #include "kernels_linalg.hpp"
+
+// Scratch of the allocation-free residual norms (residual_norms_noalloc):
+// kReduceBlocks partial 5-vectors, 40 KiB per GPU, held until the process
+// ends.  Host code never passes this name to a kernel: it is a host-side
+// stand-in.  The device address comes from hipGetSymbolAddress, once per
+// device, with that device current (noalloc_scratch below).
+static __device__ __align__(256) double scratch[cfdg::kReduceBlocks * 5];
 
 namespace cfdg {
 
@@ -511,6 +519,19 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(int nblocks, int nco
     }
 }
 
+// The partial sums of k_res_sq added in k_reduce_partials' order, then
+// l2[v] = sqrt(sum_v * scale) (solver.hpp:740-756) written to device memory.
+__global__ void __launch_bounds__(kBlock) k_norms_finish(int nblocks, const double* partial, double scale,
+                                                         double* l2) {
+    __shared__ double lds[kBlock / 64];
+    for (int j = 0; j < 5; ++j) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b * 5 + j];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) l2[j] = sqrt(s * scale);
+    }
+}
+
 // b = -R (AoS), the implicit right-hand side (implicit_euler.hpp:81-92)
 __global__ void __launch_bounds__(kBlock) k_rhs(int nc, int ld, const double* res, double* b) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -707,6 +728,46 @@ void launch_res_sq(int nc, int ld, const double* res, double* partial, double* o
     const int nb = reduction_blocks(nc);
     hipLaunchKernelGGL(k_res_sq, dim3(nb), dim3(kBlock), 0, s, nc, ld, res, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, 5, partial, out, nullptr);
+}
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up on the first
+// call with that device current and kept per device (each GPU holds its own
+// copy of the array at its own address).
+hipError_t noalloc_scratch(double** out) {
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t residual_norms_noalloc(int nc, int ld, const double* res, double* l2, hipStream_t s) {
+    double* partial = nullptr;
+    hipError_t e = noalloc_scratch(&partial);
+    if (e != hipSuccess) return e;
+    const int nb = reduction_blocks(nc);
+    const double scale = 1.0 / static_cast<double>(nc > 1 ? nc : 1);
+    hipLaunchKernelGGL(k_res_sq, dim3(nb), dim3(kBlock), 0, s, nc, ld, res, partial);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_norms_finish, dim3(1), dim3(kBlock), 0, s, nb, partial, scale, l2);
+    return hipGetLastError();
 }
 
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s) {

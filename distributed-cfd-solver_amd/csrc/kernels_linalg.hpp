@@ -80,6 +80,10 @@ template <class T>
 void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,
                         hipStream_t s);
 void launch_res_sq(int nc, int ld, const double* res, double* partial, double* out, hipStream_t s);
+// l2 (5 doubles, device) = residual norms of res (5 planes of ld, nc cells,
+// device), with the partial sums in the library's static scratch array: no
+// device allocation, no host synchronisation; stream-ordered on s.
+hipError_t residual_norms_noalloc(int nc, int ld, const double* res, double* l2, hipStream_t s);
 void launch_rhs(int nc, int ld, const double* res, double* b, hipStream_t s);
 void launch_state_update(int nc, int ld, const double* du, double* U, hipStream_t s);
 

@@ -498,6 +498,33 @@ int cfdg_stage_times(cfdg_ctx* ctx, double ms[6]);
    stream and returns the mean device time per launch in milliseconds. */
 int cfdg_time_flux_kernel(cfdg_ctx* ctx, int32_t reps, double* ms_per_launch);
 
+/* ---- allocation-free entry point -------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory: code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc.  Computes what cfdg_residual_norms computes
+   (Solver::residual_norms, solver.hpp:740-756: l2[v] = sqrt(sum_c R_v^2 /
+   n_cells)) for a residual the caller holds in device memory, variable v of
+   cell c at res[v * ld + c], and writes the five norms to l2 (device memory).
+   It shares the kernels of cfdg_residual_norms; their partial sums go to a
+   static device array of the library (1024 x 5 doubles = 40 KiB per GPU,
+   held until the process ends) instead of a context buffer, so the call
+   allocates nothing.
+   - Works on the current device, stream-ordered on `stream` (a hipStream_t;
+     NULL = the null stream), and returns without synchronising: read l2 after
+     the stream has reached it.
+   - Every call on one GPU shares that GPU's scratch array; calls that overlap
+     on one GPU (different streams or threads, or graph replays in flight
+     together) race and must be serialised by the caller.
+   - The first call on each device looks up the array's device address; a
+     caller that captures the call into a graph makes one call on that device
+     before capturing.
+   - n_cells = 0 gives zeros.  Sizes: 0 <= n_cells <= CFDG_NOALLOC_MAX_CELLS
+     (larger -> CFDG_ERR_CAPACITY, nothing launched); n_cells <= ld <=
+     INT32_MAX; NULL res or l2 -> CFDG_ERR_ARG. */
+#define CFDG_ERR_CAPACITY      -7          /* input larger than a fixed capacity */
+#define CFDG_NOALLOC_MAX_CELLS 1073741824  /* 2^30 cells */
+int cfdg_residual_norms_noalloc(const double* res, int64_t ld, int32_t n_cells, double* l2, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
