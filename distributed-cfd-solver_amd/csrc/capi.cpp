@@ -755,6 +755,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
             }
             c.face_vec.upload(fv, s);
             c.face_vec_base.upload(fvb, s);
+            c.sync();  // fv and fvb are released at the end of this block
         }
         c.sync();
     }
@@ -802,6 +803,24 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.row_ptr.upload(rp, s);
         c.cols.upload(cols, s);
         c.diag.upload(dg, s);
+        // off-diagonal block of every interior cell face, in the device cell ->
+        // face lists (the Jacobian kernel's store positions)
+        {
+            const cfdg::Csr cf_dev = to_device_rows(cf_mesh, c.new2old, nullptr);
+            std::vector<i32> blk(cf_dev.idx.size(), -1);
+            for (int r = 0; r < c.nc; ++r)
+                for (i32 e = cf_dev.off[z(r)]; e < cf_dev.off[z(r) + 1]; ++e) {
+                    const i32 f = cf_dev.idx[z(e)];
+                    if (f >= c.ninner) continue;
+                    const i32 o = owner_dev[z(f)] == r ? other_dev[z(f)] : owner_dev[z(f)];
+                    i32 k = rp[z(r)];
+                    while (k < rp[z(r) + 1] && cols[z(k)] != o) ++k;
+                    if (k == rp[z(r) + 1]) throw CfdgError(CFDG_ERR_MESH, "face neighbour missing from the BSR row");
+                    blk[z(e)] = k;
+                }
+            c.cf_blk.upload(blk, s);
+            c.sync();  // blk is released at the end of this block
+        }
         i32 ncol = 0;
         const auto col = cfdg::greedy_colours(c.ccf_mesh, c.nc, ncol);
         upload_groups(c, 0, col, ncol, false);

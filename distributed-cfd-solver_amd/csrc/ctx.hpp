@@ -107,7 +107,7 @@ struct cfdg_ctx {
     // BSR: mesh-order pattern for the ABI, device-order pattern for the kernels
     std::vector<std::int32_t> h_row_ptr, h_cols, h_diag;
     std::vector<long long> blk_dev;  // mesh block k -> device block
-    DBuf<int> row_ptr, cols, diag;
+    DBuf<int> row_ptr, cols, diag, cf_blk;
     // block values and diagonal LU as raw bytes in the storage of the precision form
     DBuf<std::uint8_t> values, lu;
     int storage = cfdg::kStoreF64;
@@ -241,6 +241,7 @@ struct cfdg_ctx {
         a.row_ptr = row_ptr.get();
         a.cols = cols.get();
         a.diag = diag.get();
+        a.cf_blk = cf_blk.get();
         return a;
     }
     void sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }

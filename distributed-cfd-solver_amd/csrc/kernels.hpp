@@ -187,6 +187,10 @@ struct BsrArgs {
     const int* row_ptr;
     const int* cols;
     const int* diag;
+    // per entry of the cell -> face lists (MeshArgs::cf_off order): the block
+    // (row = the cell, column = the face's other cell) of an interior face, -1
+    // at a boundary face
+    const int* cf_blk;
 };
 
 // Bijective block -> work-item map that gives each XCD one contiguous range of

@@ -14,7 +14,8 @@
 //   k_flux              tiled flux residual: a workgroup owns a tile of cells,
 //                       evaluates each face touching the tile once into LDS,
 //                       then every cell sums its faces in ascending face index
-//   k_dt, k_jacobian    local time step; Picard Jacobian, one thread per block row
+//   k_dt, k_jacobian    local time step; Picard Jacobian on the flux tiles, one
+//                       thread per block row
 //
 // No floating-point atomics: every output element has a single writer and
 // every sum has a fixed order.
@@ -818,73 +819,137 @@ __device__ __forceinline__ void store_block<bf16>(bf16* dst, const double v[25],
     }
 }
 
-__device__ __forceinline__ int find_col(const BsrArgs& b, int row, int col) {
-    int k = b.row_ptr[row];
-    const int e = b.row_ptr[row + 1];
-    while (k < e && b.cols[k] != col) ++k;
-    return k;  // the pattern holds every face neighbour
-}
-
-// First-order Picard Jacobian, one thread per block row (jacobian_kernel.hpp:
-// 74-270): all blocks of row c come from the thread of row c; the diagonal
-// block accumulates over the cell's faces in ascending face index.
-template <class TS>
-__global__ void __launch_bounds__(kBlock) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
-                                                     const double* alpha, TS* values) {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (c >= m.nc) return;
+// Picard Jacobian on the flux tiles (jacobian_kernel.hpp:74-270): one
+// workgroup per tile, one thread per block row.  The tile's cells' primitive
+// states, centroids (and eddy viscosities) are staged in LDS with coalesced
+// loads; each of a row's faces is read from the tile face-list stream (normal,
+// area, centroid, packed cell ids: contiguous per tile), so only the halo or
+// ghost side of a face on the tile boundary is gathered from the cell planes.
+// Row c forms, per face in ascending face index, the face's dF of its own
+// side (added to the diagonal: + dFL as owner, - dFR as neighbour) and, at an
+// interior face, the other side's dF, stored as the off-diagonal block at the
+// position precomputed at set-up (cf_blk): + dFR as owner, - dFL as neighbour.
+// The arithmetic and its order are those of the reference's face loop, so the
+// blocks are the reference's bit for bit; each block has a single writer.
+template <class TS, bool kTurb>
+__global__ void __launch_bounds__(kTileCells) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
+                                                         const double* alpha, TS* values) {
+    __shared__ double sq[8 * kTileCells];  // q (0-4) and centroid (5-7) of the tile's cells, one plane each
+    __shared__ double smut[kTurb ? kTileCells : 1];
+    __shared__ int scf[kTileCells + 1 + 2 * kTileFaces];  // the tile's cf_off (rebased), then its cf_slot entries
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
+    const int fb = m.tile_face_off[t];
+    const int i = threadIdx.x;
+    const int e0 = m.cf_off[c0];
+    if (i < nt) {
+        typedef double pair_t __attribute__((ext_vector_type(2)));
+        for (int p = 0; p < 4; ++p) {
+            const pair_t v = *reinterpret_cast<const pair_t*>(f.cs + plane_at(2 * p, f.ld, c0 + i));
+            sq[(2 * p) * kTileCells + i] = v.x;
+            sq[(2 * p + 1) * kTileCells + i] = v.y;
+        }
+        if (kTurb) smut[i] = f.mut[c0 + i];
+    }
+    {
+        // nt + 1 offsets (up to kTileCells + 1, one more than the threads)
+        const int ne = m.cf_off[c0 + nt] - e0;  // at most two entries per tile face
+        for (int k = i; k <= nt; k += kTileCells) scf[k] = m.cf_off[c0 + k] - e0;
+        for (int e = i; e < ne; e += kTileCells) scf[kTileCells + 1 + e] = m.cf_slot[e0 + e];
+    }
+    __syncthreads();
+    if (i >= nt) return;
+    const int c = c0 + i;
+    double qc[5], Uc[5];
+    for (int v = 0; v < 5; ++v) qc[v] = sq[v * kTileCells + i];
+    prim_to_cons(nm.gas, qc, Uc);
+    const double xc0 = sq[5 * kTileCells + i], xc1 = sq[6 * kTileCells + i], xc2 = sq[7 * kTileCells + i];
+    const double mc = kTurb ? smut[i] : 0.0;
     double dg[25];
-    for (int i = 0; i < 25; ++i) dg[i] = 0.0;
-    for (int e = m.cf_off[c]; e < m.cf_off[c + 1]; ++e) {
-        const int fc = m.cf[e];
-        const FaceRef fr = frec(m, fc);
-        const bool inner = fc < m.ninner;
-        const int c0 = face_owner(m, fc);
-        const int c1 = face_other(m, fc);
-        const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
-        const CellRef r0 = crec(f, c0);
-        const CellRef r1 = crec(f, c1);
+    for (int k = 0; k < 25; ++k) dg[k] = 0.0;
+    for (int e = scf[i]; e < scf[i + 1]; ++e) {
+        const int sl = scf[kTileCells + 1 + e];
+        const bool own = sl >= 0;
+        const int j = fb + (own ? sl : -sl - 1);
+        typedef double pair_t __attribute__((ext_vector_type(2)));
+        const pair_t n01 = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN, m.tld, j));
+        const pair_t n2a = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN + 2, m.tld, j));
+        const pair_t zid = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kTCells - 1, m.tld, j));
+        const double nx = n01.x, ny = n01.y, nz = n2a.x, A = n2a.y;
+        const unsigned long long ids = static_cast<unsigned long long>(__double_as_longlong(zid.y));
+        const int cx = static_cast<int>(static_cast<unsigned>(ids));
+        const int cy = static_cast<int>(static_cast<unsigned>(ids >> 32));
+        const bool inner = cy < m.nc;  // boundary faces: the other cell is a ghost
+        const int other = own ? cy : cx;
+        const unsigned so = static_cast<unsigned>(other - c0);
+        const bool oin = so < static_cast<unsigned>(nt);
+        double qo[5], xo[3];
+        double mo = 0.0;
+        if (oin) {
+            for (int v = 0; v < 5; ++v) qo[v] = sq[v * kTileCells + so];
+            for (int d = 0; d < 3; ++d) xo[d] = sq[(5 + d) * kTileCells + so];
+            if (kTurb) mo = smut[so];
+        } else {
+            const CellRef r = crec(f, other);
+            for (int v = 0; v < 5; ++v) qo[v] = r[kRQ + v];
+            for (int d = 0; d < 3; ++d) xo[d] = r[kRC + d];
+            if (kTurb) mo = f.mut[other];
+        }
+        double Uo[5];
+        prim_to_cons(nm.gas, qo, Uo);
         double qL[5], qR[5], UL[5], UR[5];
         for (int v = 0; v < 5; ++v) {
-            qL[v] = r0[kRQ + v];
-            qR[v] = r1[kRQ + v];
+            qL[v] = own ? qc[v] : qo[v];
+            qR[v] = own ? qo[v] : qc[v];
+            UL[v] = own ? Uc[v] : Uo[v];
+            UR[v] = own ? Uo[v] : Uc[v];
         }
-        prim_to_cons(nm.gas, qL, UL);
-        prim_to_cons(nm.gas, qR, UR);
         const double smax = hllc_smax(nm.gas, UL, UR, nx, ny, nz);
         const double half_area = 0.5 * A;
         const double diss = 0.5 * smax * A;
         double vm[9];
-        // jacobian_kernel.hpp:156-157, 209-210, 252-254: the ghost side carries no eddy viscosity
-        const double mutL = f.mut ? f.mut[c0] : 0.0;
-        const double mutR = (f.mut && inner) ? f.mut[c1] : 0.0;
-        if (nm.viscous) viscous_jacobian_entries(nm.gas, fr[kFDinv], qL, qR, mutL, mutR, A, vm);
+        if (nm.viscous) {
+            // owner -> neighbour centroid vector and its inverse length with the
+            // operations of face_distances (aux_geometry.cpp:11-51, 120-156)
+            double inv_d;
+            if (inner) {
+                const double ax = own ? xc0 : xo[0], ay = own ? xc1 : xo[1], az = own ? xc2 : xo[2];
+                const double bx = own ? xo[0] : xc0, by = own ? xo[1] : xc1, bz = own ? xo[2] : xc2;
+                const double ex = bx - ax, ey = by - ay, ez = bz - az;
+                inv_d = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+            } else {  // the row cell owns a boundary face
+                const pair_t fxy = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFC, m.tld, j));
+                const double ex = fxy.x - xc0, ey = fxy.y - xc1, ez = zid.x - xc2;
+                inv_d = 1.0 / sqrt(4.0 * ex * ex + 4.0 * ey * ey + 4.0 * ez * ez);
+            }
+            // jacobian_kernel.hpp:156-157, 209-210, 252-254: the ghost side carries no eddy viscosity
+            const double mutL = kTurb ? (own ? mc : mo) : 0.0;
+            const double mutR = (kTurb && inner) ? (own ? mo : mc) : 0.0;
+            viscous_jacobian_entries(nm.gas, inv_d, qL, qR, mutL, mutR, A, vm);
+        }
         double J[25], Jv[25];
-        const bool own = (c0 == c);
-        // dFL = 0.5 A J(UL) + diss I (+ viscous), dFR = 0.5 A J(UR) - diss I (- viscous);
-        // row c: diagonal += dFL (owner) or -= dFR (neighbour); off-diagonal =
-        // dFR (owner) or -dFL (neighbour)
-        for (int pass = 0; pass < (inner ? 2 : 1); ++pass) {
-            const bool left = (pass == 0) ? own : !own;
-            euler_jacobian_scaled(nm.gas, left ? UL : UR, nx, ny, nz, half_area, J);
-            for (int i = 0; i < 5; ++i) J[i * 6] = left ? (J[i * 6] + diss) : (J[i * 6] - diss);
-            if (nm.viscous) {
-                viscous_jacobian_dense(vm, Jv);
-                for (int i = 0; i < 25; ++i) J[i] += left ? Jv[i] : -Jv[i];
-            }
-            if (pass == 0) {
-                if (own)
-                    for (int i = 0; i < 25; ++i) dg[i] += J[i];
-                else
-                    for (int i = 0; i < 25; ++i) dg[i] -= J[i];
-            } else {
-                const int k = find_col(b, c, own ? c1 : c0);
-                store_block(values + static_cast<long long>(k) * BlockLayout<TS>::block, J, own ? 1 : -1);
-            }
+        if (nm.viscous) viscous_jacobian_dense(vm, Jv);
+        // own side: diagonal += dFL (owner) or -= dFR (neighbour), with
+        // dFL = 0.5 A J(UL) + diss I (+ viscous), dFR = 0.5 A J(UR) - diss I (- viscous)
+        euler_jacobian_scaled(nm.gas, Uc, nx, ny, nz, half_area, J);
+        for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] + diss) : (J[k * 6] - diss);
+        if (nm.viscous)
+            for (int k = 0; k < 25; ++k) J[k] += own ? Jv[k] : -Jv[k];
+        if (own)
+            for (int k = 0; k < 25; ++k) dg[k] += J[k];
+        else
+            for (int k = 0; k < 25; ++k) dg[k] -= J[k];
+        if (inner) {  // other side: off-diagonal block = dFR (owner) or -dFL (neighbour)
+            euler_jacobian_scaled(nm.gas, Uo, nx, ny, nz, half_area, J);
+            for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] - diss) : (J[k * 6] + diss);
+            if (nm.viscous)
+                for (int k = 0; k < 25; ++k) J[k] += own ? -Jv[k] : Jv[k];
+            const int kb = b.cf_blk[e0 + e];
+            store_block(values + static_cast<long long>(kb) * BlockLayout<TS>::block, J, own ? 1 : -1);
         }
     }
     const double a = alpha[c];
-    for (int i = 0; i < 5; ++i) dg[i * 6] += a;
+    for (int k = 0; k < 5; ++k) dg[k * 6] += a;
     store_block(values + static_cast<long long>(b.diag[c]) * BlockLayout<TS>::block, dg, 0);
 }
 
@@ -1007,15 +1072,21 @@ void launch_unsteady_source(const MeshArgs& m, const FieldArgs& f, int bdf, doub
 void launch_jacobian(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, const BsrArgs& b,
                      const double* alpha, void* values, int storage, hipStream_t s) {
     if (m.nc <= 0) return;
-    if (storage == kStoreF32)
-        hipLaunchKernelGGL(k_jacobian<float>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
-                           static_cast<float*>(values));
-    else if (storage == kStoreBF16)
-        hipLaunchKernelGGL(k_jacobian<bf16>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
-                           static_cast<bf16*>(values));
-    else
-        hipLaunchKernelGGL(k_jacobian<double>, grid_for(m.nc), dim3(kBlock), 0, s, m, f, nm, b, alpha,
-                           static_cast<double*>(values));
+    const dim3 grid(static_cast<unsigned>(m.ntiles)), block(kTileCells);
+    const bool turb = f.mut != nullptr;
+    if (storage == kStoreF32) {
+        float* v = static_cast<float*>(values);
+        if (turb) hipLaunchKernelGGL((k_jacobian<float, true>), grid, block, 0, s, m, f, nm, b, alpha, v);
+        else hipLaunchKernelGGL((k_jacobian<float, false>), grid, block, 0, s, m, f, nm, b, alpha, v);
+    } else if (storage == kStoreBF16) {
+        bf16* v = static_cast<bf16*>(values);
+        if (turb) hipLaunchKernelGGL((k_jacobian<bf16, true>), grid, block, 0, s, m, f, nm, b, alpha, v);
+        else hipLaunchKernelGGL((k_jacobian<bf16, false>), grid, block, 0, s, m, f, nm, b, alpha, v);
+    } else {
+        double* v = static_cast<double*>(values);
+        if (turb) hipLaunchKernelGGL((k_jacobian<double, true>), grid, block, 0, s, m, f, nm, b, alpha, v);
+        else hipLaunchKernelGGL((k_jacobian<double, false>), grid, block, 0, s, m, f, nm, b, alpha, v);
+    }
 }
 
 }  // namespace cfdg
