@@ -643,7 +643,7 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         const int p_hi = (nm.muscl || nm.viscous) ? kPairs : 3;
         if (sl < nt) {
             for (int p = g; p < p_hi; p += kGroups) {
-                if (p > 2 && p < 7 && !nm.muscl) continue;
+                if (p > 3 && p < 7 && !nm.muscl) continue;  // centroid (pairs 2-3) stays: viscous terms need it
                 const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(2 * p, f.ld, c0 + sl));
                 *reinterpret_cast<double2*>(sc + (p * kTileCells + sl) * 2) = v;
             }
@@ -819,6 +819,151 @@ __device__ __forceinline__ void store_block<bf16>(bf16* dst, const double v[25],
     }
 }
 
+// Block of the storage type TS as storage words (8 bytes for FP64, 4 for FP32
+// and BF16), the unit of the Jacobian kernel's coalesced block stores.
+template <class TS>
+struct BlockWords {
+    typedef std::uint32_t word;
+    static constexpr int n = BlockLayout<TS>::block * static_cast<int>(sizeof(TS)) / 4;
+};
+template <>
+struct BlockWords<double> {
+    typedef unsigned long long word;
+    static constexpr int n = 25;
+};
+
+// The words of one block as store_block writes it (sign: +1 -> 0 + v,
+// -1 -> 0 - v, 0 -> v); a BF16 block's padding word is zero.
+template <class TS>
+__device__ __forceinline__ void block_words(const double v[25], int sign, typename BlockWords<TS>::word* w);
+template <>
+__device__ __forceinline__ void block_words<double>(const double v[25], int sign, unsigned long long* w) {
+    for (int k = 0; k < 25; ++k) {
+        const double x = sign > 0 ? 0.0 + v[k] : (sign < 0 ? 0.0 - v[k] : v[k]);
+        w[k] = static_cast<unsigned long long>(__double_as_longlong(x));
+    }
+}
+template <>
+__device__ __forceinline__ void block_words<float>(const double v[25], int sign, std::uint32_t* w) {
+    for (int k = 0; k < 25; ++k) {
+        const double x = sign > 0 ? 0.0 + v[k] : (sign < 0 ? 0.0 - v[k] : v[k]);
+        w[k] = f32_bits(cvt<float>(x));
+    }
+}
+template <>
+__device__ __forceinline__ void block_words<bf16>(const double v[25], int sign, std::uint32_t* w) {
+    for (int a = 0; a < 5; ++a) {
+        std::uint32_t h[6];
+        for (int q = 0; q < 5; ++q) {
+            const double x = sign > 0 ? 0.0 + v[a * 5 + q] : (sign < 0 ? 0.0 - v[a * 5 + q] : v[a * 5 + q]);
+            h[q] = cvt<bf16>(x).u;
+        }
+        h[5] = 0;
+        for (int q = 0; q < 3; ++q) w[a * 3 + q] = h[2 * q] | (h[2 * q + 1] << 16);
+    }
+    w[15] = 0;
+}
+
+// J += Jv (plus) or J -= Jv for the dense viscous block Jv of
+// viscous_jacobian_dense, touching only its 11 nonzero entries.  Adding the
+// pattern's zeros would change no stored value: it can only turn a -0 entry
+// into +0, and the diagonal sums start at +0 while every off-diagonal store
+// forms 0 + v or 0 - v.
+__device__ __forceinline__ void add_viscous_jacobian(const double m[9], bool plus, double J[25]) {
+    const double d[11] = {m[0], m[3], m[1], m[3], m[2], m[3], m[4], m[5], m[6], m[7], m[8]};
+    const int at[11] = {5, 6, 10, 12, 15, 18, 20, 21, 22, 23, 24};
+    for (int k = 0; k < 11; ++k) J[at[k]] = plus ? J[at[k]] + d[k] : J[at[k]] - d[k];
+}
+
+// One face of a Jacobian row (jacobian_kernel.hpp:104-270 for one face): the
+// row cell's state from LDS, the face from the tile face stream, the other
+// cell from LDS (tile cell) or the cell planes (halo cell, ghost).  Adds the
+// face's dF of the row's own side to dg (+ dFL as owner, - dFR as
+// neighbour); at an interior face returns true with the other side's dF in J
+// (dFR as owner, dFL as neighbour: the caller stores + J or - J).
+template <bool kTurb>
+__device__ __forceinline__ bool jacobian_face(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm,
+                                              const double* sq, const double* smut, int i, int c0, int nt, int j,
+                                              bool own, double dg[25], double J[25]) {
+    typedef double pair_t __attribute__((ext_vector_type(2)));
+    // the row cell's state, re-read from LDS per face (fewer live registers)
+    double qc[5], Uc[5];
+    for (int v = 0; v < 5; ++v) qc[v] = sq[v * kTileCells + i];
+    prim_to_cons(nm.gas, qc, Uc);
+    const pair_t n01 = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN, m.tld, j));
+    const pair_t n2a = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN + 2, m.tld, j));
+    const pair_t zid = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kTCells - 1, m.tld, j));
+    const double nx = n01.x, ny = n01.y, nz = n2a.x, A = n2a.y;
+    const unsigned long long ids = static_cast<unsigned long long>(__double_as_longlong(zid.y));
+    const int cx = static_cast<int>(static_cast<unsigned>(ids));
+    const int cy = static_cast<int>(static_cast<unsigned>(ids >> 32));
+    const bool inner = cy < m.nc;  // boundary faces: the other cell is a ghost
+    const int oc = own ? cy : cx;
+    const unsigned so = static_cast<unsigned>(oc - c0);
+    const bool oin = so < static_cast<unsigned>(nt);
+    double qo[5], xo[3];
+    double mo = 0.0;
+    if (oin) {
+        for (int v = 0; v < 5; ++v) qo[v] = sq[v * kTileCells + so];
+        for (int d = 0; d < 3; ++d) xo[d] = sq[(5 + d) * kTileCells + so];
+        if (kTurb) mo = smut[so];
+    } else {
+        const CellRef r = crec(f, oc);
+        for (int v = 0; v < 5; ++v) qo[v] = r[kRQ + v];
+        for (int d = 0; d < 3; ++d) xo[d] = r[kRC + d];
+        if (kTurb) mo = f.mut[oc];
+    }
+    double Uo[5];
+    prim_to_cons(nm.gas, qo, Uo);
+    double qL[5], qR[5], UL[5], UR[5];
+    for (int v = 0; v < 5; ++v) {
+        qL[v] = own ? qc[v] : qo[v];
+        qR[v] = own ? qo[v] : qc[v];
+        UL[v] = own ? Uc[v] : Uo[v];
+        UR[v] = own ? Uo[v] : Uc[v];
+    }
+    const double smax = hllc_smax(nm.gas, UL, UR, nx, ny, nz);
+    const double half_area = 0.5 * A;
+    const double diss = 0.5 * smax * A;
+    double vm[9];
+    if (nm.viscous) {
+        // owner -> neighbour centroid vector and its inverse length with the
+        // operations of face_distances (aux_geometry.cpp:11-51, 120-156)
+        const double xc0 = sq[5 * kTileCells + i], xc1 = sq[6 * kTileCells + i], xc2 = sq[7 * kTileCells + i];
+        double inv_d;
+        if (inner) {
+            const double ax = own ? xc0 : xo[0], ay = own ? xc1 : xo[1], az = own ? xc2 : xo[2];
+            const double bx = own ? xo[0] : xc0, by = own ? xo[1] : xc1, bz = own ? xo[2] : xc2;
+            const double ex = bx - ax, ey = by - ay, ez = bz - az;
+            inv_d = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+        } else {  // the row cell owns a boundary face
+            const pair_t fxy = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFC, m.tld, j));
+            const double ex = fxy.x - xc0, ey = fxy.y - xc1, ez = zid.x - xc2;
+            inv_d = 1.0 / sqrt(4.0 * ex * ex + 4.0 * ey * ey + 4.0 * ez * ez);
+        }
+        // jacobian_kernel.hpp:156-157, 209-210, 252-254: the ghost side carries no eddy viscosity
+        const double mc = kTurb ? smut[i] : 0.0;
+        const double mutL = kTurb ? (own ? mc : mo) : 0.0;
+        const double mutR = (kTurb && inner) ? (own ? mo : mc) : 0.0;
+        viscous_jacobian_entries(nm.gas, inv_d, qL, qR, mutL, mutR, A, vm);
+    }
+    // own side, dFL = 0.5 A J(UL) + diss I (+ viscous) as owner, dFR = 0.5 A
+    // J(UR) - diss I (- viscous) as neighbour
+    euler_jacobian_scaled(nm.gas, Uc, nx, ny, nz, half_area, J);
+    for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] + diss) : (J[k * 6] - diss);
+    if (nm.viscous) add_viscous_jacobian(vm, own, J);
+    if (own)
+        for (int k = 0; k < 25; ++k) dg[k] += J[k];
+    else
+        for (int k = 0; k < 25; ++k) dg[k] -= J[k];
+    if (!inner) return false;
+    // other side: dFR as owner, dFL as neighbour
+    euler_jacobian_scaled(nm.gas, Uo, nx, ny, nz, half_area, J);
+    for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] - diss) : (J[k * 6] + diss);
+    if (nm.viscous) add_viscous_jacobian(vm, !own, J);
+    return true;
+}
+
 // Picard Jacobian on the flux tiles (jacobian_kernel.hpp:74-270): one
 // workgroup per tile, one thread per block row.  The tile's cells' primitive
 // states, centroids (and eddy viscosities) are staged in LDS with coalesced
@@ -827,21 +972,32 @@ __device__ __forceinline__ void store_block<bf16>(bf16* dst, const double v[25],
 // ghost side of a face on the tile boundary is gathered from the cell planes.
 // Row c forms, per face in ascending face index, the face's dF of its own
 // side (added to the diagonal: + dFL as owner, - dFR as neighbour) and, at an
-// interior face, the other side's dF, stored as the off-diagonal block at the
-// position precomputed at set-up (cf_blk): + dFR as owner, - dFL as neighbour.
+// interior face, the other side's dF, the off-diagonal block at the position
+// precomputed at set-up (cf_blk): + dFR as owner, - dFL as neighbour.
+// The rows advance through their faces in lock step (round r = each row's
+// r-th face, then a last round for the diagonal blocks); each round's blocks
+// go through LDS, and each wave stores its 64 blocks word by word with
+// consecutive lanes on consecutive words, so a store instruction writes a few
+// contiguous runs instead of 64 scattered words.
 // The arithmetic and its order are those of the reference's face loop, so the
 // blocks are the reference's bit for bit; each block has a single writer.
 template <class TS, bool kTurb>
 __global__ void __launch_bounds__(kTileCells) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
                                                          const double* alpha, TS* values) {
+    typedef typename BlockWords<TS>::word word;
+    constexpr int kW = BlockWords<TS>::n;
     __shared__ double sq[8 * kTileCells];  // q (0-4) and centroid (5-7) of the tile's cells, one plane each
     __shared__ double smut[kTurb ? kTileCells : 1];
     __shared__ int scf[kTileCells + 1 + 2 * kTileFaces];  // the tile's cf_off (rebased), then its cf_slot entries
+    __shared__ word sblk[kTileCells * kW];  // one block per row for the round's store
+    __shared__ int spos[kTileCells];        // its destination block, -1 for none
+    __shared__ int srounds;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = m.tile_off[t], nt = m.tile_off[t + 1] - c0;
     const int fb = m.tile_face_off[t];
     const int i = threadIdx.x;
     const int e0 = m.cf_off[c0];
+    if (i == 0) srounds = 0;
     if (i < nt) {
         typedef double pair_t __attribute__((ext_vector_type(2)));
         for (int p = 0; p < 4; ++p) {
@@ -858,99 +1014,45 @@ __global__ void __launch_bounds__(kTileCells) k_jacobian(MeshArgs m, FieldArgs f
         for (int e = i; e < ne; e += kTileCells) scf[kTileCells + 1 + e] = m.cf_slot[e0 + e];
     }
     __syncthreads();
-    if (i >= nt) return;
-    const int c = c0 + i;
-    double qc[5], Uc[5];
-    for (int v = 0; v < 5; ++v) qc[v] = sq[v * kTileCells + i];
-    prim_to_cons(nm.gas, qc, Uc);
-    const double xc0 = sq[5 * kTileCells + i], xc1 = sq[6 * kTileCells + i], xc2 = sq[7 * kTileCells + i];
-    const double mc = kTurb ? smut[i] : 0.0;
+    const bool row = i < nt;
+    const int nfc = row ? scf[i + 1] - scf[i] : 0;
+    if (nfc > 0) atomicMax(&srounds, nfc);  // integer maximum: the tile's longest face list
+    const int c = c0 + (row ? i : 0);
     double dg[25];
     for (int k = 0; k < 25; ++k) dg[k] = 0.0;
-    for (int e = scf[i]; e < scf[i + 1]; ++e) {
-        const int sl = scf[kTileCells + 1 + e];
-        const bool own = sl >= 0;
-        const int j = fb + (own ? sl : -sl - 1);
-        typedef double pair_t __attribute__((ext_vector_type(2)));
-        const pair_t n01 = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN, m.tld, j));
-        const pair_t n2a = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFN + 2, m.tld, j));
-        const pair_t zid = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kTCells - 1, m.tld, j));
-        const double nx = n01.x, ny = n01.y, nz = n2a.x, A = n2a.y;
-        const unsigned long long ids = static_cast<unsigned long long>(__double_as_longlong(zid.y));
-        const int cx = static_cast<int>(static_cast<unsigned>(ids));
-        const int cy = static_cast<int>(static_cast<unsigned>(ids >> 32));
-        const bool inner = cy < m.nc;  // boundary faces: the other cell is a ghost
-        const int other = own ? cy : cx;
-        const unsigned so = static_cast<unsigned>(other - c0);
-        const bool oin = so < static_cast<unsigned>(nt);
-        double qo[5], xo[3];
-        double mo = 0.0;
-        if (oin) {
-            for (int v = 0; v < 5; ++v) qo[v] = sq[v * kTileCells + so];
-            for (int d = 0; d < 3; ++d) xo[d] = sq[(5 + d) * kTileCells + so];
-            if (kTurb) mo = smut[so];
-        } else {
-            const CellRef r = crec(f, other);
-            for (int v = 0; v < 5; ++v) qo[v] = r[kRQ + v];
-            for (int d = 0; d < 3; ++d) xo[d] = r[kRC + d];
-            if (kTurb) mo = f.mut[other];
-        }
-        double Uo[5];
-        prim_to_cons(nm.gas, qo, Uo);
-        double qL[5], qR[5], UL[5], UR[5];
-        for (int v = 0; v < 5; ++v) {
-            qL[v] = own ? qc[v] : qo[v];
-            qR[v] = own ? qo[v] : qc[v];
-            UL[v] = own ? Uc[v] : Uo[v];
-            UR[v] = own ? Uo[v] : Uc[v];
-        }
-        const double smax = hllc_smax(nm.gas, UL, UR, nx, ny, nz);
-        const double half_area = 0.5 * A;
-        const double diss = 0.5 * smax * A;
-        double vm[9];
-        if (nm.viscous) {
-            // owner -> neighbour centroid vector and its inverse length with the
-            // operations of face_distances (aux_geometry.cpp:11-51, 120-156)
-            double inv_d;
-            if (inner) {
-                const double ax = own ? xc0 : xo[0], ay = own ? xc1 : xo[1], az = own ? xc2 : xo[2];
-                const double bx = own ? xo[0] : xc0, by = own ? xo[1] : xc1, bz = own ? xo[2] : xc2;
-                const double ex = bx - ax, ey = by - ay, ez = bz - az;
-                inv_d = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
-            } else {  // the row cell owns a boundary face
-                const pair_t fxy = *reinterpret_cast<const pair_t*>(m.tf + plane_at(kFC, m.tld, j));
-                const double ex = fxy.x - xc0, ey = fxy.y - xc1, ez = zid.x - xc2;
-                inv_d = 1.0 / sqrt(4.0 * ex * ex + 4.0 * ey * ey + 4.0 * ez * ez);
+    __syncthreads();
+    const int rounds = srounds;
+    const int lane = i & 63, wbase = i & ~63;
+    for (int rr = 0; rr <= rounds; ++rr) {
+        int pos = -1;
+        if (rr < nfc) {
+            const int e = scf[i] + rr;
+            const int sl = scf[kTileCells + 1 + e];
+            const bool own = sl >= 0;
+            double J[25];
+            if (jacobian_face<kTurb>(m, f, nm, sq, smut, i, c0, nt, fb + (own ? sl : -sl - 1), own, dg, J)) {
+                pos = b.cf_blk[e0 + e];
+                block_words<TS>(J, own ? 1 : -1, sblk + i * kW);
             }
-            // jacobian_kernel.hpp:156-157, 209-210, 252-254: the ghost side carries no eddy viscosity
-            const double mutL = kTurb ? (own ? mc : mo) : 0.0;
-            const double mutR = (kTurb && inner) ? (own ? mo : mc) : 0.0;
-            viscous_jacobian_entries(nm.gas, inv_d, qL, qR, mutL, mutR, A, vm);
+        } else if (rr == rounds && row) {  // last round: the diagonal block
+            const double a = alpha[c];
+            for (int k = 0; k < 5; ++k) dg[k * 6] += a;
+            pos = b.diag[c];
+            block_words<TS>(dg, 0, sblk + i * kW);
         }
-        double J[25], Jv[25];
-        if (nm.viscous) viscous_jacobian_dense(vm, Jv);
-        // own side: diagonal += dFL (owner) or -= dFR (neighbour), with
-        // dFL = 0.5 A J(UL) + diss I (+ viscous), dFR = 0.5 A J(UR) - diss I (- viscous)
-        euler_jacobian_scaled(nm.gas, Uc, nx, ny, nz, half_area, J);
-        for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] + diss) : (J[k * 6] - diss);
-        if (nm.viscous)
-            for (int k = 0; k < 25; ++k) J[k] += own ? Jv[k] : -Jv[k];
-        if (own)
-            for (int k = 0; k < 25; ++k) dg[k] += J[k];
-        else
-            for (int k = 0; k < 25; ++k) dg[k] -= J[k];
-        if (inner) {  // other side: off-diagonal block = dFR (owner) or -dFL (neighbour)
-            euler_jacobian_scaled(nm.gas, Uo, nx, ny, nz, half_area, J);
-            for (int k = 0; k < 5; ++k) J[k * 6] = own ? (J[k * 6] - diss) : (J[k * 6] + diss);
-            if (nm.viscous)
-                for (int k = 0; k < 25; ++k) J[k] += own ? -Jv[k] : Jv[k];
-            const int kb = b.cf_blk[e0 + e];
-            store_block(values + static_cast<long long>(kb) * BlockLayout<TS>::block, J, own ? 1 : -1);
+        spos[i] = pos;
+        __syncthreads();
+        // this wave's 64 blocks, word by word: consecutive lanes on consecutive words
+        word* out = reinterpret_cast<word*>(values);
+#pragma unroll 1
+        for (int q = 0; q < kW; ++q) {
+            const int w = q * 64 + lane;
+            const int blk = w / kW, o = w - blk * kW;
+            const int dst = spos[wbase + blk];
+            if (dst >= 0) out[static_cast<long long>(dst) * kW + o] = sblk[(wbase + blk) * kW + o];
         }
+        __syncthreads();  // the next round reuses sblk / spos
     }
-    const double a = alpha[c];
-    for (int k = 0; k < 5; ++k) dg[k * 6] += a;
-    store_block(values + static_cast<long long>(b.diag[c]) * BlockLayout<TS>::block, dg, 0);
 }
 
 // Per-patch boundary integrals (residual_kernel.hpp:96-169 forms F[0] and

@@ -12,7 +12,7 @@ physical steps of dual time stepping (BDF1, BDF2) from the uniform state;
 the vtu_* fixtures the reference's solution files after a few implicit steps;
 plate_mut the residual and Jacobian with a prescribed eddy-viscosity field.
 
-usage: python tests/golden/make_golden.py [time|vtu|mut|rcm|linsys]
+usage: python tests/golden/make_golden.py [time|vtu|mut|rcm|linsys|<fixture name>]
 """
 import os
 import sys
@@ -62,6 +62,11 @@ def configs():
     c = cases.bump(ni=3, nj=3, nk=3)
     out.append(("bump_tet_lsq", c, 3))
     out.append(("bc_zoo", bc_zoo(), 3))
+    # viscous first order: gradients only for the viscous terms, no limiter
+    c = cases.plate(ni=8, nj=6, nk=4, reconstruction=cfdg.FIRST_ORDER)
+    out.append(("plate_first_order", c, 3))
+    c = cases.bump(ni=3, nj=3, nk=3, reconstruction=cfdg.FIRST_ORDER, gradient="GREEN_GAUSS_FACE")
+    out.append(("bump_first_order", c, 2))
     return out
 
 
@@ -299,8 +304,8 @@ if __name__ == "__main__":
     if not ref_run.available():
         sys.exit("reference harness missing: run oracle/build_ref.sh first")
     only = sys.argv[1] if len(sys.argv) > 1 else ""
-    if not only:
-        for name, case, steps in configs():
+    for name, case, steps in configs():
+        if not only or only == name:
             make(name, case, steps)
     if only in ("", "rcm"):
         make_rcm()
