@@ -9,8 +9,11 @@ growth 1.08, +-10 % seeded interior jitter, reverse Cuthill-McKee order,
 Mach 0.3, Re_L = 1e5, FP64.  Inputs are resident in HBM before timing.
 
 Prints ONE JSON line.  Extra fields: the flux kernel's roofline, the per-stage
-device times, one implicit step (Jacobian + BiCGSTAB/SGS) and the CPU baseline
-(the reference's own compiled code where available, else the C++ oracle port).
+device times, implicit steps (Jacobian + BiCGSTAB/SGS) in the three precision
+forms, the CPU baseline (the reference's own compiled code where available,
+else the C++ oracle port), the time to a 1e-8 residual drop on a 262,144-cell
+plate in four forms (time_to_drop) and the 20.25M-tet transonic SA bump:
+its residual and the forms timed to an equal drop (config5).
 
 usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5]
 """
@@ -42,10 +45,11 @@ def parse():
     ap.add_argument("--drop-previous-update", action="store_true",
                     help="also time the previous-update guess alone (the network with all parameters zero)")
     ap.add_argument("--drop-max-steps", type=int, default=12000)
-    ap.add_argument("--tet-sa", default="60",
-                    help="hexahedra per direction of the tetrahedral SA bump residual (each split into 6 tets), or off")
-    ap.add_argument("--sa", choices=["on", "off"], default="on",
-                    help="also time the residual with the Spalart-Allmaras module on the same mesh")
+    ap.add_argument("--config5", default="150",
+                    help="hexahedra per direction of the transonic SA tetrahedral bump (each split into 6 Kuhn "
+                         "tetrahedra; 150 = the 20.25M-tet case), or off")
+    ap.add_argument("--config5-steps", type=int, default=100,
+                    help="implicit steps of the plain FP64 run that fix the drop the other forms are timed to")
     ap.add_argument("--drop-profile", type=int, default=0, metavar="N",
                     help="profiling mode: N implicit steps of the time-to-drop workload in each form, nothing else")
     ap.add_argument("--pmc-run", action="store_true",
@@ -208,20 +212,33 @@ def main():
         "setup_s": {"mesh": round(t_mesh, 2), "context": round(t_setup, 2)},
     }
     solver.close()
-    if a.sa != "off":
-        out["sa_residual"] = sa_residual(a)
-    if a.tet_sa != "off":
-        out["tet_sa_residual"] = tet_sa_residual(a, int(a.tet_sa))
+    gnn = None
     if a.drop_size != "off":
-        out["time_to_drop"] = time_to_drop(a)
+        out["time_to_drop"], gnn = time_to_drop(a)
+    if a.config5 != "off":
+        out["config5"] = config5(a, int(a.config5), gnn)
     print(json.dumps(out))
 
 
-def tet_sa_residual(a, n):
-    """Residual evaluation of the transonic tetrahedral bump with the
-    Spalart-Allmaras module (Mach 0.8, Re 1e6; n^3 hexahedra split into 6
-    Kuhn tetrahedra each, 5 % node jitter): the wide LDS gradient form, the
-    tiled flux and module kernels on 70-neighbour stencils."""
+def sa_flux_algorithmic_bytes(nc, nbf, nf):
+    # the viscous count plus the eddy viscosity of cells and ghosts (8 B each)
+    return flux_algorithmic_bytes(nc, nbf, nf, viscous=True) + 8 * (nc + nbf)
+
+
+def config5(a, n, gnn):
+    """The transonic RANS bump: n^3 hexahedra split into 6 Kuhn tetrahedra
+    each (n = 150: 20.25M tets), 5 % seeded node jitter, Gaussian bump of
+    height 0.1 and width 0.25, Mach 0.8, Re 1e6, viscous + Spalart-Allmaras
+    with the reference's frozen module update, node LSQ + Venkatakrishnan
+    MUSCL, CFL 10, from the freestream with 2 % seeded noise.
+
+    - residual: one evaluation with the module (primitives, ghosts, LSQ
+      gradients + limiter and the nu_tilde gradient, ghost gradients, flux
+      and module residual), median of 10; the flux kernel's roofline;
+    - forms: this case does not reach a 1e-8 drop under any setting of the
+      reference (its relative residual stalls near 1e-2), so the plain FP64
+      run takes a fixed number of implicit steps and every other form is
+      timed to the drop that run reached, from the same state."""
     import numpy as np
 
     import cfdg
@@ -230,66 +247,75 @@ def tet_sa_residual(a, n):
     case = cases.bump_sa(ni=n, nj=n, nk=n)
     t0 = time.perf_counter()
     mesh = cfdg.Mesh.generate(**case.mesh)
+    t_mesh = time.perf_counter() - t0
+    print(f"bench.py: config5 mesh {mesh.n_cells} cells in {t_mesh:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
     s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    t_ctx = time.perf_counter() - t0
     s.init_uniform(case.rho, case.velocity, case.p)
-    U = s.get_state()
-    s.set_state(U * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal(U.shape)))
-    s.set_turbulence(cases.turbulence(case))
-    t_set = time.perf_counter() - t0
-    for _ in range(a.warmup):
+    U0 = s.get_state() * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal((5, mesh.n_cells)))
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    s.set_turbulence(cases.turbulence(case, update=cfdg.TURB_FROZEN))
+    t_turb = time.perf_counter() - t0
+    print(f"bench.py: config5 context {t_ctx:.1f} s, turbulence {t_turb:.1f} s", file=sys.stderr, flush=True)
+    rnu0 = s.turbulence()["rho_nu"]
+    nc, nf, nbf = mesh.n_cells, mesh.n_faces, mesh.n_faces - mesh.n_inner_faces
+    for _ in range(2):
         s.eval_residual()
     t = []
-    for _ in range(max(a.steps // 5, 5)):
+    for _ in range(10):
         t1 = time.perf_counter()
         s.eval_residual()
         t.append(time.perf_counter() - t1)
     ms = 1e3 * float(np.median(t))
     stages = s.stage_times()
     flux_ms = s.time_flux_kernel(10)
-    nc = mesh.n_cells
+    B = sa_flux_algorithmic_bytes(nc, nbf, nf)
+    out = {"workload": f"transonic bump, {n}x{n}x{n} hexahedra x 6 tetrahedra ({nc} cells, {nf} faces), "
+                       "Mach 0.8, Re 1e6, viscous + Spalart-Allmaras (frozen module), node LSQ + "
+                       "Venkatakrishnan MUSCL, CFL 10, 2% seeded noise",
+           "setup_s": {"mesh": round(t_mesh, 1), "context": round(t_ctx, 1), "turbulence": round(t_turb, 1)},
+           "residual": {"ms": round(ms, 3), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
+                        "stage_ms": [round(float(x), 3) for x in stages[:4]]},
+           "flux_roofline": {"kernel": "k_flux<true> (flux residual with eddy viscosity)",
+                             "ms_per_launch": round(flux_ms, 4), "algorithmic_bytes": B,
+                             "achieved": round(B / (flux_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                             "frac": round(B / (flux_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    print(f"bench.py: config5 residual {out['residual']}", file=sys.stderr, flush=True)
+    lp = cases.default_linparams()
+    forms = [("plain_fp64", cfdg.Precision.make(), None),
+             ("fp32_krylov", cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1), None),
+             ("bf16_blocks_fp32_krylov", cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=8,
+                                                             inner_rel_tol=0.1), None)]
+    if gnn is not None:
+        forms.append(("fp32_krylov_learned", cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8,
+                                                                 inner_rel_tol=0.1), gnn))
+    target = None
+    N = a.config5_steps
+    for name, prec, net in forms:
+        s.set_precision(prec)
+        s.set_state(U0)
+        s.set_turbulence_state(rnu0)
+        s.set_warm_start(net)
+        if target is None:
+            r = s.run_steady(lp, N, rel_tol=0.0, check_interval=1)
+            target = r.rel
+        else:
+            r = s.run_steady(lp, 4 * N, rel_tol=target, check_interval=1)
+        row = {"seconds": round(r.seconds, 3), "steps": int(r.steps), "linear_iterations": int(r.linear_iterations),
+               "rel": r.rel, "reached": bool(r.rel <= target), "ms_per_step": round(1e3 * r.seconds / max(r.steps, 1), 2)}
+        if net is not None:
+            row["warm_start_kept_rejected"] = list(s.warm_start_stats())
+        out[name] = row
+        print(f"bench.py: config5 {name}: {row}", file=sys.stderr, flush=True)
+    s.set_warm_start(None)
+    out["drop"] = target
+    t_plain = out["plain_fp64"]["seconds"]
+    for name, _, _ in forms[1:]:
+        out[f"speedup_{name}"] = round(t_plain / out[name]["seconds"], 3)
     s.close()
-    return {"workload": f"transonic bump, {n}x{n}x{n} hexahedra x 6 tetrahedra ({nc} cells), Mach 0.8, Re 1e6, "
-                        "viscous + Spalart-Allmaras (point-implicit module), node LSQ + Venkatakrishnan MUSCL",
-            "ms": round(ms, 4), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
-            "stage_ms": [round(float(x), 4) for x in stages[:4]], "flux_kernel_ms": round(flux_ms, 4),
-            "setup_s": round(t_set, 2)}
-
-
-def sa_residual(a):
-    """Residual evaluation with the Spalart-Allmaras module on the plate mesh
-    at Re_L = 5e6: the mean-flow residual with eddy viscosity plus the
-    module's ghosts, gradient, face sweep and sources, device-resident."""
-    import numpy as np
-
-    import cfdg
-    import cfdg.cases as cases
-
-    ni, nj, nk = (int(x) for x in a.size.split("x"))
-    case = cases.plate_sa(ni=ni, nj=nj, nk=nk)
-    mesh = cfdg.Mesh.generate(**case.mesh)
-    s = cfdg.Solver(mesh, case.numerics, case.bcs)
-    s.init_uniform(case.rho, case.velocity, case.p)
-    U = s.get_state()
-    s.set_state(U * (1.0 + 0.02 * np.random.default_rng(1234).standard_normal(U.shape)))
-    t0 = time.perf_counter()
-    s.set_turbulence(cases.turbulence(case))
-    t_set = time.perf_counter() - t0
-    for _ in range(a.warmup):
-        s.eval_residual()
-    t = []
-    for _ in range(max(a.steps // 5, 5)):
-        t1 = time.perf_counter()
-        s.eval_residual()
-        t.append(time.perf_counter() - t1)
-    ms = 1e3 * float(np.median(t))
-    stages = s.stage_times()
-    nc = mesh.n_cells
-    s.close()
-    return {"workload": f"flat plate {ni}x{nj}x{nk} ({nc} cells), Re_L 5e6, viscous + Spalart-Allmaras "
-                        "(point-implicit module), node LSQ + Venkatakrishnan MUSCL",
-            "ms": round(ms, 4), "mcells_s": round(nc / (ms * 1e-3) / 1e6, 1),
-            "stage_ms": [round(float(x), 4) for x in stages[:4]],
-            "set_turbulence_s": round(t_set, 2)}
+    return out
 
 
 def stream_ceiling(nbytes=1 << 31, reps=10):
@@ -375,15 +401,18 @@ def time_to_drop(a):
     """Wall time to a 1e-8 drop of the reference's relative residual
     (max_v L2_v / L2_v of the first step, solver.hpp:740-765) on a steady
     laminar plate, for the plain FP64 path, the two mixed forms (FP32 or BF16
-    blocks, FP32 Krylov inside FP64 defect correction) and the BF16 form with
-    the learned warm start.  Second-order unlimited MUSCL with the
-    node-LSQ gradient (the Venkatakrishnan limiter stalls this case near 1e-6),
-    CFL 1000, from the freestream with seeded 2 % noise.  The network is
-    trained first on 8 seeded 32x16x8 plates with Mach drawn from U(0.2, 0.5)
-    (every 100th of 2000 steps recorded, so the samples span the whole
-    convergence history); its training time is reported separately.  With
-    --drop-previous-update the previous-update guess alone (the network with
-    all parameters zero) is timed too, so the network's own share shows."""
+    blocks, FP32 Krylov inside FP64 defect correction) and the FP32 form with
+    the learned warm start.  Second-order unlimited MUSCL with the node-LSQ
+    gradient, CFL 1000, from the freestream with seeded 2 % noise; the
+    Venkatakrishnan-limited scheme of the 2.1M-cell plate stalls this case
+    near 5e-7 (limiter cycling), so it cannot be timed to 1e-8.  The network
+    is trained first on 8 seeded 32x16x8 plates with Mach drawn from
+    U(0.2, 0.5) (every 100th of 2000 steps recorded); its training time is
+    reported separately.  After the timed run every form continues, untimed,
+    by a further 1e-4 drop (to about 1e-12 overall), and its converged
+    primitive fields and wall force are compared with the plain FP64 run's
+    continued the same way.  With --drop-previous-update the previous-update
+    guess alone (the network with all parameters zero) is timed too."""
     import numpy as np
 
     import cfdg
@@ -403,8 +432,16 @@ def time_to_drop(a):
         s.set_state(U * (1.0 + 0.02 * np.random.default_rng(seed).standard_normal(U.shape)))
         return s
 
+    case = cases.plate(ni=ni, nj=nj, nk=nk, **kw)
+    walls = [p for p, b in enumerate(case.bcs) if b.type == cfdg.BC["NO_SLIP_WALL_HEAT_FLUX"]]
+
+    def primitives(U):
+        rho = U[0]
+        u, v, w = U[1] / rho, U[2] / rho, U[3] / rho
+        p = (cases.GAMMA - 1.0) * (U[4] - 0.5 * rho * (u * u + v * v + w * w))
+        return np.stack([rho, u, v, w, p])
+
     def run(form, gnn=None):
-        case = cases.plate(ni=ni, nj=nj, nk=nk, **kw)
         s = start(case, 1)
         if form != "plain":
             s.set_precision(forms[form])
@@ -415,25 +452,34 @@ def time_to_drop(a):
                "converged": r.status == cfdg.CONVERGED, "rel": r.rel}
         if gnn is not None:
             res["warm_start_kept_rejected"] = list(s.warm_start_stats())
-        U = s.get_state()
+        # untimed: a further 1e-4 drop, then the converged fields and wall force
+        c = s.run_steady(lp, 8000, rel_tol=1e-4)
+        res["continued"] = {"steps": int(c.steps), "reached": c.status == cfdg.CONVERGED,
+                            "rel_overall_approx": r.rel * c.rel}
+        s.eval_residual()
+        _, _, force = s.boundary_integrals()
+        q = primitives(s.get_state())
         s.close()
-        return res, U
+        return res, (q, force[walls].sum(axis=0))
 
     out = {"workload": f"laminar plate {ni}x{nj}x{nk} hexahedra, Mach 0.3, Re_L 1e5, unlimited MUSCL + node LSQ, "
                        "CFL 1000, BiCGSTAB + multicolour SGS (rel 0.1), 2% seeded noise on the freestream"}
-    out["plain_fp64"], U0 = run("plain")
-    out["fp32_krylov"], U1 = run("fp32")
-    out["bf16_blocks_fp32_krylov"], U3 = run("bf16")
+    out["plain_fp64"], ref = run("plain")
+    print(f"bench.py: drop plain_fp64 {out['plain_fp64']}", file=sys.stderr, flush=True)
+    conv = {}
+    out["fp32_krylov"], conv["fp32_krylov"] = run("fp32")
+    out["bf16_blocks_fp32_krylov"], conv["bf16_blocks_fp32_krylov"] = run("bf16")
+    print(f"bench.py: drop mixed forms done", file=sys.stderr, flush=True)
     if a.drop_previous_update:
         zero = cfdg.Gnn(seed=1)
         zero.set_params(np.zeros(zero.num_params, np.float32))
-        out["bf16_blocks_previous_update"], _ = run("bf16", zero)
+        out["fp32_krylov_previous_update"], _ = run("fp32", zero)
     t0 = time.perf_counter()
     gnn = cfdg.Gnn(seed=11)
     rng = np.random.default_rng(2024)
     for m in range(8):
-        case = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m, **kw)
-        s = start(case, m)
+        tcase = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m, **kw)
+        s = start(tcase, m)
         s.set_recording(True)
         for it in range(2000):
             s.implicit_step(lp)
@@ -443,23 +489,50 @@ def time_to_drop(a):
     loss = gnn.train(epochs=30, lr=1e-3, seed=5)
     out["training"] = {"seconds": round(time.perf_counter() - t0, 2), "samples": gnn.num_samples(),
                        "final_loss": loss}
-    out["bf16_blocks_fp32_krylov_learned"], U2 = run("bf16", gnn)
-    scale = np.abs(U0).max(axis=1, keepdims=True)
-    out["state_error_vs_fp64"] = {"fp32_krylov": float(np.max(np.abs(U1 - U0) / scale)),
-                                  "bf16_blocks_fp32_krylov": float(np.max(np.abs(U3 - U0) / scale)),
-                                  "bf16_blocks_fp32_krylov_learned": float(np.max(np.abs(U2 - U0) / scale))}
-    t0 = out["plain_fp64"]["seconds"]
-    out["speedup_mixed_fp32"] = round(t0 / out["fp32_krylov"]["seconds"], 3)
-    out["speedup_mixed_bf16"] = round(t0 / out["bf16_blocks_fp32_krylov"]["seconds"], 3)
-    out["speedup_mixed_learned"] = round(t0 / out["bf16_blocks_fp32_krylov_learned"]["seconds"], 3)
-    return out
+    out["fp32_krylov_learned"], conv["fp32_krylov_learned"] = run("fp32", gnn)
+    # converged fields against the plain FP64 run's: rho, p against the
+    # freestream values, u, v, w against the freestream speed; wall force relative
+    speed = float(np.linalg.norm(case.velocity))
+    scale = np.array([case.rho, speed, speed, speed, case.p])[:, None]
+    q0, f0 = ref
+    out["converged_error_vs_fp64"] = {
+        name: {"rho": float(np.max(np.abs(q[0] - q0[0]) / scale[0])), "u": float(np.max(np.abs(q[1] - q0[1]) / scale[1])),
+               "v": float(np.max(np.abs(q[2] - q0[2]) / scale[2])), "w": float(np.max(np.abs(q[3] - q0[3]) / scale[3])),
+               "p": float(np.max(np.abs(q[4] - q0[4]) / scale[4])),
+               "wall_force": float(np.linalg.norm(f - f0) / np.linalg.norm(f0))}
+        for name, (q, f) in conv.items()}
+    t_plain = out["plain_fp64"]["seconds"]
+    out["speedup_mixed_fp32"] = round(t_plain / out["fp32_krylov"]["seconds"], 3)
+    out["speedup_mixed_bf16"] = round(t_plain / out["bf16_blocks_fp32_krylov"]["seconds"], 3)
+    out["speedup_mixed_fp32_learned"] = round(t_plain / out["fp32_krylov_learned"]["seconds"], 3)
+    return out, gnn
+
+
+def host_cpu():
+    """Host cores and CPU model of the machine the bench runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "logical_cpus": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(a, mesh, case, U, nc):
     """The reference's Solver::evaluate_residual (compiled from its own sources,
-    oracle/_ref) on the same mesh and state, or the C++ oracle port."""
+    oracle/_ref) on the same mesh and state, or the C++ oracle port; one rank
+    on one host core (the host's core count is recorded beside it)."""
     import numpy as np
     cores = 1
+    host = host_cpu()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     if a.cpu_baseline in ("auto", "reference"):
         try:
@@ -471,7 +544,7 @@ def cpu_baseline(a, mesh, case, U, nc):
                 return {"value": round(nc / t / 1e6, 4), "unit": "Mcells/s", "cores": cores, "kind": "reference",
                         "sample": f"{len(times)} residual evaluations of the full {nc}-cell mesh by the reference's "
                                   "Solver::evaluate_residual, 1 MPI rank, median",
-                        "s_per_eval": round(t, 3)}
+                        "s_per_eval": round(t, 3), "host": host}
         except Exception as e:  # noqa: BLE001  (fall back to the port and say so)
             print(f"bench.py: reference CPU baseline unavailable ({e}); timing the oracle port", file=sys.stderr)
         if a.cpu_baseline == "reference":
@@ -487,7 +560,7 @@ def cpu_baseline(a, mesh, case, U, nc):
     t = float(np.median(times))
     return {"value": round(nc / t / 1e6, 4), "unit": "Mcells/s", "cores": cores, "kind": "port",
             "sample": f"{len(times)} residual evaluations of the full {nc}-cell mesh by the C++ oracle, median",
-            "s_per_eval": round(t, 3)}
+            "s_per_eval": round(t, 3), "host": host}
 
 
 if __name__ == "__main__":

@@ -710,11 +710,9 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         // them), else consecutive sub-ranges of them that fit the wide form
         const cfdg::Csr ccf_dev = to_device_rows(c.ccf_mesh, c.new2old, &c.old2new);
         std::vector<i32> goff = tl.tile_off;
-        cfdg::TileStencil ts = cfdg::tile_stencil(goff, st, ccf_dev);
-        if (ts.max_slots > cfdg::kGradSlotsBig) {
+        if (cfdg::tile_max_slots(goff, st, ccf_dev) > cfdg::kGradSlotsBig)
             goff = cfdg::split_tiles(tl.tile_off, st, ccf_dev, cfdg::kGradSlotsBig);
-            ts = cfdg::tile_stencil(goff, st, ccf_dev);
-        }
+        const cfdg::TileStencil ts = cfdg::tile_stencil(goff, st, ccf_dev);
         c.gslots = (ts.max_slots <= cfdg::kGradSlots) ? cfdg::kGradSlots : cfdg::kGradSlotsBig;
         if (ts.max_slots <= cfdg::kGradSlotsBig) {
             int most = 0;
@@ -1276,8 +1274,8 @@ int cfdg_set_unsteady(cfdg_ctx* c, double dt, int32_t order) {
     return guarded(c, [&] {
         if (order < 0 || order > 2 || (order > 0 && !(dt > 0.0)))
             throw CfdgError(CFDG_ERR_ARG, "dual time stepping needs order 0, 1 or 2 and dt > 0");
-        if (order > 0 && c->sa_on())
-            throw CfdgError(CFDG_ERR_CONFIG, "dual time stepping with the turbulence module is not supported");
+        if (order > 0 && c->sa_on() && c->turb.update != CFDG_TURB_FROZEN)
+            throw CfdgError(CFDG_ERR_CONFIG, "dual time stepping supports the turbulence module only with the frozen update");
         c->bdf_order = order;
         c->phys_dt = dt;
         c->bdf_cur = 1;
@@ -1360,8 +1358,10 @@ int cfdg_boundary_integrals(cfdg_ctx* c, double* mass, double* energy, double* f
 int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulence* t) {
     return guarded(c, [&] {
         if (!t) throw CfdgError(CFDG_ERR_ARG, "null turbulence parameters");
-        if (t->model != CFDG_TURB_NONE && c->bdf_order > 0)
-            throw CfdgError(CFDG_ERR_CONFIG, "the turbulence module is not supported in dual time stepping");
+        // in dual time stepping the module block is copied unchanged, as the
+        // reference's implicit Euler does (implicit_euler.hpp:64-72)
+        if (t->model != CFDG_TURB_NONE && c->bdf_order > 0 && t->update != CFDG_TURB_FROZEN)
+            throw CfdgError(CFDG_ERR_CONFIG, "dual time stepping supports the turbulence module only with the frozen update");
         c->have_residual = false;
         c->have_dt = false;
         if (t->model == CFDG_TURB_NONE) {

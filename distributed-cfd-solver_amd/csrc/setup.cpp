@@ -1,8 +1,13 @@
 #include "setup.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
 #include <stdexcept>
+#include <thread>
+
+#include "parallel.hpp"
 
 namespace cfdg {
 
@@ -60,23 +65,30 @@ Csr cell_cells_by_node(const Mesh& m) {
             for (i32 k = m.cell_nodes_off[z(c)]; k < m.cell_nodes_off[z(c) + 1]; ++k)
                 nc.idx[z(cur[z(m.cell_nodes[z(k)])]++)] = c;
     }
-    Csr out;
-    out.off.assign(z(m.n_cells) + 1, 0);
-    std::vector<i32> stamp(z(m.n_cells), -1);
-    for (i32 c = 0; c < m.n_cells; ++c) {
-        const std::size_t start = out.idx.size();
+    // per cell: the cells of its nodes, sorted, without duplicates and itself
+    std::vector<std::vector<i32>> scratch(z(parallel_workers()));
+    const auto gather = [&](i32 c, std::vector<i32>& v) {
+        v.clear();
         for (i32 k = m.cell_nodes_off[z(c)]; k < m.cell_nodes_off[z(c) + 1]; ++k) {
             const i32 n = m.cell_nodes[z(k)];
-            for (i32 j = nc.off[z(n)]; j < nc.off[z(n) + 1]; ++j) {
-                const i32 cand = nc.idx[z(j)];
-                if (cand == c || stamp[z(cand)] == c) continue;
-                stamp[z(cand)] = c;
-                out.idx.push_back(cand);
-            }
+            for (i32 j = nc.off[z(n)]; j < nc.off[z(n) + 1]; ++j)
+                if (nc.idx[z(j)] != c) v.push_back(nc.idx[z(j)]);
         }
-        std::sort(out.idx.begin() + static_cast<std::ptrdiff_t>(start), out.idx.end());
-        out.off[z(c) + 1] = static_cast<i32>(out.idx.size());
-    }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    };
+    Csr out;
+    out.off.assign(z(m.n_cells) + 1, 0);
+    parallel_for(z(m.n_cells), [&](std::size_t c, int w) {
+        gather(static_cast<i32>(c), scratch[z(w)]);
+        out.off[c + 1] = static_cast<i32>(scratch[z(w)].size());
+    });
+    for (i32 c = 0; c < m.n_cells; ++c) out.off[z(c) + 1] += out.off[z(c)];
+    out.idx.resize(z(out.off.back()));
+    parallel_for(z(m.n_cells), [&](std::size_t c, int w) {
+        gather(static_cast<i32>(c), scratch[z(w)]);
+        std::copy(scratch[z(w)].begin(), scratch[z(w)].end(), out.idx.begin() + out.off[c]);
+    });
     return out;
 }
 
@@ -265,16 +277,23 @@ struct Bisector {
     const Csr& ccf;
     const Csr& cf;
     int max_cells, max_faces;
-    std::vector<i32> cells, work, best;
-    std::vector<i32> stamp;  // per cell: tag of the set it was last marked in
+    std::vector<i32>& cells;  // shared: every bisector works on its own range
+    std::vector<i32> work, best;
+    std::vector<i32> stamp;   // per cell: tag of the set it was last marked in
     i32 tag = 0;
     std::vector<i32> tile_off;
+
+    Bisector(const Mesh& mm, const Csr& a, const Csr& b, int mc, int mf, std::vector<i32>& c)
+        : m(mm), ccf(a), cf(b), max_cells(mc), max_faces(mf), cells(c), stamp(c.size(), 0) {}
 
     double coord(i32 c, int a) const {
         return a == 0 ? m.cell_cx[z(c)] : (a == 1 ? m.cell_cy[z(c)] : m.cell_cz[z(c)]);
     }
 
-    void split(std::size_t lo, std::size_t hi) {
+    // The two halves of a split are independent: down to depth 4 the right
+    // half runs on a thread of its own (with its own marks), and its tiles are
+    // appended after the left half's, so the tiling is the sequential one.
+    void split(std::size_t lo, std::size_t hi, int depth = 0) {
         const std::size_t n = hi - lo;
         const std::size_t cap = static_cast<std::size_t>(max_cells);
         if (n <= cap && tile_face_count(cf, m, cells, lo, hi, stamp, ++tag) <= max_faces) {
@@ -307,8 +326,29 @@ struct Bisector {
             }
         }
         std::copy(best.begin(), best.end(), cells.begin() + static_cast<long>(lo));
-        split(lo, lo + nl);
-        split(lo + nl, hi);
+        if (depth < 4 && n > (std::size_t{1} << 16)) {
+            Bisector right(m, ccf, cf, max_cells, max_faces, cells);
+            std::exception_ptr err;
+            std::thread th([&] {
+                try {
+                    right.split(lo + nl, hi, depth + 1);
+                } catch (...) {
+                    err = std::current_exception();
+                }
+            });
+            try {
+                split(lo, lo + nl, depth + 1);
+            } catch (...) {
+                th.join();
+                throw;
+            }
+            th.join();
+            if (err) std::rethrow_exception(err);
+            tile_off.insert(tile_off.end(), right.tile_off.begin(), right.tile_off.end());
+        } else {
+            split(lo, lo + nl, depth + 1);
+            split(lo + nl, hi, depth + 1);
+        }
     }
 };
 
@@ -317,16 +357,16 @@ struct Bisector {
 Tiling tile_cells(const Mesh& m, const Csr& ccf, const Csr& cf, int max_cells, int max_faces) {
     const i32 n = m.n_cells;
     Tiling t;
-    Bisector b{m, ccf, cf, max_cells, max_faces, {}, {}, {}, {}, 0, {}};
-    b.cells.resize(z(n));
-    for (i32 c = 0; c < n; ++c) b.cells[z(c)] = c;
-    b.stamp.assign(z(n), 0);
+    std::vector<i32> cells(z(n));
+    for (i32 c = 0; c < n; ++c) cells[z(c)] = c;
+    Bisector b(m, ccf, cf, max_cells, max_faces, cells);
     b.tile_off.push_back(0);
     if (n > 0) b.split(0, z(n));
     t.tile_off = std::move(b.tile_off);
-    t.new2old = std::move(b.cells);
-    for (std::size_t k = 0; k + 1 < t.tile_off.size(); ++k)
+    t.new2old = std::move(cells);
+    parallel_for(t.tile_off.size() - 1, [&](std::size_t k, int) {
         std::sort(t.new2old.begin() + t.tile_off[k], t.new2old.begin() + t.tile_off[k + 1]);
+    });
     t.old2new.assign(z(n), -1);
     for (i32 d = 0; d < n; ++d) t.old2new[z(t.new2old[z(d)])] = d;
     return t;
@@ -369,10 +409,44 @@ TileFaces tile_faces(const Tiling& t, const Csr& dev_cf, const std::vector<i32>&
     return tf;
 }
 
+namespace {
+
+// Halo of a tile [c0, c1): the cells outside it that its cells' node and face
+// stencils reach, ascending.
+void tile_halo(i32 c0, i32 c1, const Csr& dev_ccn, const Csr& dev_ccf, std::vector<i32>& local) {
+    local.clear();
+    for (i32 c = c0; c < c1; ++c) {
+        for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
+            const i32 nb = dev_ccn.idx[z(e)];
+            if (nb < c0 || nb >= c1) local.push_back(nb);
+        }
+        for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) {
+            const i32 nb = dev_ccf.idx[z(e)];
+            if (nb < c0 || nb >= c1) local.push_back(nb);
+        }
+    }
+    std::sort(local.begin(), local.end());
+    local.erase(std::unique(local.begin(), local.end()), local.end());
+}
+
+}  // namespace
+
+i32 tile_max_slots(const std::vector<i32>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf) {
+    const std::size_t ntiles = tile_off.size() - 1;
+    std::vector<i32> slots(ntiles, 0);
+    std::vector<std::vector<i32>> scratch(z(parallel_workers()));
+    parallel_for(ntiles, [&](std::size_t k, int w) {
+        tile_halo(tile_off[k], tile_off[k + 1], dev_ccn, dev_ccf, scratch[z(w)]);
+        slots[k] = (tile_off[k + 1] - tile_off[k]) + static_cast<i32>(scratch[z(w)].size());
+    }, 4);
+    i32 most = 0;
+    for (const i32 v : slots) most = std::max(most, v);
+    return most;
+}
+
 TileStencil tile_stencil(const std::vector<i32>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf) {
     TileStencil ts;
     const std::size_t ntiles = tile_off.size() - 1;
-    const std::size_t n = dev_ccn.off.size() - 1;
     const std::vector<i32>& toff = tile_off;
     ts.halo_off.assign(ntiles + 1, 0);
     ts.ccn_base.assign(ntiles + 1, 0);
@@ -389,59 +463,61 @@ TileStencil tile_stencil(const std::vector<i32>& tile_off, const Csr& dev_ccn, c
     }
     ts.ccn_slot.assign(z(ts.ccn_base[ntiles]), 0);
     ts.ccf_slot.assign(z(ts.ccf_base[ntiles]), 0);
-    std::vector<i32> slot(n, -1), local;
+    // halo of every tile (independent per tile)
+    std::vector<std::vector<i32>> halos(ntiles);
+    parallel_for(ntiles, [&](std::size_t k, int) { tile_halo(toff[k], toff[k + 1], dev_ccn, dev_ccf, halos[k]); }, 4);
     for (std::size_t k = 0; k < ntiles; ++k) {
-        const i32 c0 = toff[k], c1 = toff[k + 1];
-        local.clear();
-        for (i32 c = c0; c < c1; ++c) {
-            for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
-                const i32 nb = dev_ccn.idx[z(e)];
-                if (nb < c0 || nb >= c1) local.push_back(nb);
-            }
-            for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) {
-                const i32 nb = dev_ccf.idx[z(e)];
-                if (nb < c0 || nb >= c1) local.push_back(nb);
-            }
-        }
-        std::sort(local.begin(), local.end());
-        local.erase(std::unique(local.begin(), local.end()), local.end());
-        const i32 nslots = (c1 - c0) + static_cast<i32>(local.size());
+        const i32 nslots = (toff[k + 1] - toff[k]) + static_cast<i32>(halos[k].size());
         if (nslots > 65535) throw std::runtime_error("tile stencil: more than 65535 LDS slots");
         ts.max_slots = std::max(ts.max_slots, nslots);
-        for (i32 c = c0; c < c1; ++c) slot[z(c)] = c - c0;
-        for (std::size_t h = 0; h < local.size(); ++h) slot[z(local[h])] = (c1 - c0) + static_cast<i32>(h);
-        const i32 nt = c1 - c0;
+        ts.halo_off[k + 1] = ts.halo_off[k] + static_cast<i32>(halos[k].size());
+    }
+    ts.halo.resize(z(ts.halo_off[ntiles]));
+    // slot lists: a tile cell is its offset in the tile, a halo cell follows
+    // the tile cells in ascending order
+    parallel_for(ntiles, [&](std::size_t k, int) {
+        const i32 c0 = toff[k], c1 = toff[k + 1], nt = c1 - c0;
+        const std::vector<i32>& local = halos[k];
+        std::copy(local.begin(), local.end(), ts.halo.begin() + ts.halo_off[k]);
+        const auto slot = [&](i32 nb) -> std::uint16_t {
+            if (nb >= c0 && nb < c1) return static_cast<std::uint16_t>(nb - c0);
+            const auto it = std::lower_bound(local.begin(), local.end(), nb);
+            return static_cast<std::uint16_t>(nt + static_cast<i32>(it - local.begin()));
+        };
         for (i32 c = c0; c < c1; ++c) {
             const i32 i = c - c0;
             for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) {
                 const i32 kk = e - dev_ccn.off[z(c)];
-                ts.ccn_slot[z(ts.ccn_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] =
-                    static_cast<std::uint16_t>(slot[z(dev_ccn.idx[z(e)])]);
+                ts.ccn_slot[z(ts.ccn_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] = slot(dev_ccn.idx[z(e)]);
             }
             for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) {
                 const i32 kk = e - dev_ccf.off[z(c)];
-                ts.ccf_slot[z(ts.ccf_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] =
-                    static_cast<std::uint16_t>(slot[z(dev_ccf.idx[z(e)])]);
+                ts.ccf_slot[z(ts.ccf_base[k] + ((kk >> 2) * nt + i) * 4 + (kk & 3))] = slot(dev_ccf.idx[z(e)]);
             }
         }
-        for (i32 c = c0; c < c1; ++c) slot[z(c)] = -1;
-        for (const i32 h : local) slot[z(h)] = -1;
-        ts.halo.insert(ts.halo.end(), local.begin(), local.end());
-        ts.halo_off[k + 1] = static_cast<i32>(ts.halo.size());
-    }
+        std::vector<i32>().swap(halos[k]);
+    }, 4);
     return ts;
 }
 
 std::vector<i32> split_tiles(const std::vector<i32>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf, int max_slots) {
     const std::size_t n = dev_ccn.off.size() - 1;
-    std::vector<i32> out{0};
-    std::vector<i32> mark(n, -1);  // chunk id that counts the cell as halo
-    i32 chunk = 0;
+    const std::size_t ntiles = tile_off.size() - 1;
     const auto for_nb = [&](i32 c, auto&& fn) {
         for (i32 e = dev_ccn.off[z(c)]; e < dev_ccn.off[z(c) + 1]; ++e) fn(dev_ccn.idx[z(e)]);
         for (i32 e = dev_ccf.off[z(c)]; e < dev_ccf.off[z(c) + 1]; ++e) fn(dev_ccf.idx[z(e)]);
     };
-    for (std::size_t k = 0; k + 1 < tile_off.size(); ++k) {
+    // per tile, independently: the cells that start a new sub-range.  Each
+    // worker marks cells with chunk ids of its own (increasing), so its mark
+    // array never needs clearing.
+    std::vector<std::vector<i32>> cuts(ntiles);
+    const int nw = parallel_workers();
+    std::vector<std::vector<i32>> marks(z(nw));
+    std::vector<i32> chunks(z(nw), 0);
+    parallel_for(ntiles, [&](std::size_t k, int w) {
+        std::vector<i32>& mark = marks[z(w)];
+        if (mark.empty()) mark.assign(n, -1);  // chunk id that counts the cell as halo
+        i32& chunk = chunks[z(w)];
         const i32 t0 = tile_off[k], t1 = tile_off[k + 1];
         i32 start = t0, halo = 0;
         ++chunk;
@@ -459,7 +535,7 @@ std::vector<i32> split_tiles(const std::vector<i32>& tile_off, const Csr& dev_cc
             });
             const i32 was_halo = (mark[z(c)] == chunk) ? 1 : 0;
             if (c > start && (c + 1 - start) + (halo - was_halo + fresh) > max_slots) {
-                out.push_back(c);  // close the chunk before c
+                cuts[k].push_back(c);  // close the chunk before c
                 start = c;
                 halo = 0;
                 ++chunk;
@@ -475,7 +551,11 @@ std::vector<i32> split_tiles(const std::vector<i32>& tile_off, const Csr& dev_cc
                 }
             });
         }
-        out.push_back(t1);
+    }, 4);
+    std::vector<i32> out{0};
+    for (std::size_t k = 0; k < ntiles; ++k) {
+        out.insert(out.end(), cuts[k].begin(), cuts[k].end());
+        out.push_back(tile_off[k + 1]);
     }
     return out;
 }

@@ -114,6 +114,8 @@ struct TileStencil {
     std::int32_t max_slots = 0;
 };
 TileStencil tile_stencil(const std::vector<std::int32_t>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf);
+// The largest slot count (cells + stencil halo) of any tile, without the slot lists.
+std::int32_t tile_max_slots(const std::vector<std::int32_t>& tile_off, const Csr& dev_ccn, const Csr& dev_ccf);
 
 // Splits each tile (a range of device cells) into consecutive sub-ranges whose
 // cells plus stencil halo take at most max_slots LDS slots, for meshes whose

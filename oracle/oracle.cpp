@@ -2074,8 +2074,8 @@ int oracle_set_unsteady(oracle_ctx* c, double dt, int32_t order) {
         c->err = "unsteady: order 0, 1 or 2 and dt > 0 required";
         return -1;
     }
-    if (order > 0 && c->sa_on()) {
-        c->err = "dual time stepping with the turbulence module is not supported";
+    if (order > 0 && c->sa_on() && c->sa.cfg.update != 0) {
+        c->err = "dual time stepping supports the turbulence module only with the frozen update";
         return -1;
     }
     c->bdf_order = order;
@@ -2254,8 +2254,8 @@ int oracle_set_eddy_viscosity(oracle_ctx* c, const double* mut, int32_t n_total)
 int oracle_set_turbulence(oracle_ctx* c, const oracle_turbulence* t) {
     if (!c || !t) return -1;
     if (t->model != 0) c->mut_ext.clear();
-    if (t->model != 0 && c->bdf_order > 0) {
-        c->err = "the turbulence module is not supported in dual time stepping";
+    if (t->model != 0 && c->bdf_order > 0 && t->update != 0) {
+        c->err = "dual time stepping supports the turbulence module only with the frozen update";
         return -1;
     }
     if (t->model == 0) {
