@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcfdg.so")
 OK = 0
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH, ERR_CAPACITY = -1, -2, -3, -4, -5, -6, -7
 NOALLOC_MAX_CELLS = 1 << 30
+GRAD_GATHER, GRAD_LDS, GRAD_LDS_WIDE = 0, 1, 2
 INVISCID_FLOW, VISCOUS_FLOW = 0, 1
 FIRST_ORDER, MUSCL = 0, 1
 LIMITER = {"VENKAT": 0, "BARTH": 1, "VAN_ALBADA": 2, "UNLIMITED": 3}
@@ -179,6 +180,7 @@ def _load():
         "cfdg_set_eddy_viscosity": [vp, vp, _i32],
         "cfdg_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "cfdg_residual_norms_noalloc": [vp, _i64, _i32, vp, vp],
+        "cfdg_gradient_form": [vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -528,6 +530,12 @@ class Solver:
         ms = np.empty(6)
         self._chk(lib.cfdg_stage_times(self._h, _ptr(ms)))
         return ms
+
+    def gradient_form(self):
+        """(form, gradient tiles, flux tiles): which gradient + limiter kernel runs."""
+        f, g, t = _i32(), _i32(), _i32()
+        self._chk(lib.cfdg_gradient_form(self._h, C.byref(f), C.byref(g), C.byref(t)))
+        return f.value, g.value, t.value
 
     def time_flux_kernel(self, reps):
         ms = _d()

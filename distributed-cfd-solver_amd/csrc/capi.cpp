@@ -1558,6 +1558,15 @@ int cfdg_time_flux_kernel(cfdg_ctx* c, int32_t reps, double* ms_per_launch) {
     });
 }
 
+int cfdg_gradient_form(const cfdg_ctx* c, int32_t* form, int32_t* n_gradient_tiles, int32_t* n_flux_tiles) {
+    if (!c) return CFDG_ERR_ARG;
+    const bool lds = c->tile_halo_off.size() > 0;
+    if (form) *form = !lds ? CFDG_GRAD_GATHER : (c->gslots > cfdg::kGradSlots ? CFDG_GRAD_LDS_WIDE : CFDG_GRAD_LDS);
+    if (n_gradient_tiles) *n_gradient_tiles = lds ? c->ngtiles : 0;
+    if (n_flux_tiles) *n_flux_tiles = c->ntiles;
+    return CFDG_OK;
+}
+
 int cfdg_residual_norms_noalloc(const double* res, int64_t ld, int32_t n_cells, double* l2, void* stream) {
     if (!res || !l2 || n_cells < 0) return CFDG_ERR_ARG;
     if (n_cells > CFDG_NOALLOC_MAX_CELLS) return CFDG_ERR_CAPACITY;

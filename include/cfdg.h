@@ -497,6 +497,17 @@ int cfdg_stage_times(cfdg_ctx* ctx, double ms[6]);
 /* Launches the flux-residual kernel reps times back to back on the context
    stream and returns the mean device time per launch in milliseconds. */
 int cfdg_time_flux_kernel(cfdg_ctx* ctx, int32_t reps, double* ms_per_launch);
+/* The form of the gradient + limiter kernel this context runs (no reference
+   counterpart; for tests and tuning).  form: CFDG_GRAD_GATHER (one thread per
+   cell gathering its stencil from the cell planes: Green-Gauss gradients),
+   CFDG_GRAD_LDS (node stencils staged in LDS, at most 640 slots per tile) or
+   CFDG_GRAD_LDS_WIDE (1088 slots).  n_gradient_tiles: workgroups of the LDS
+   forms, more than the n_flux_tiles when tiles are split into sub-ranges
+   whose stencils fit.  Any output may be NULL. */
+#define CFDG_GRAD_GATHER   0
+#define CFDG_GRAD_LDS      1
+#define CFDG_GRAD_LDS_WIDE 2
+int cfdg_gradient_form(const cfdg_ctx* ctx, int32_t* form, int32_t* n_gradient_tiles, int32_t* n_flux_tiles);
 
 /* ---- allocation-free entry point -------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate device

@@ -31,6 +31,10 @@ from oracle_py import Oracle
 pytestmark = pytest.mark.gpu
 
 CASES = small_cases()
+# the SA geometries of the precision configurations: the supersonic ramp with
+# no-slip walls at Re_L 5e6 and the transonic tetrahedral bump, frozen module
+CASES["ramp_sa"] = cases.ramp_sa()
+CASES["bump_sa"] = cases.bump_sa(ni=8, nj=8, nk=8)
 FP32 = cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=20, inner_rel_tol=0.1)
 BF16 = cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=40, inner_rel_tol=0.1)
 
@@ -41,6 +45,11 @@ def _system(name):
     gpu = cfdg.Solver(mesh, case.numerics, case.bcs)
     ora = Oracle(mesh, case.numerics, case.bcs)
     U = perturbed_state(ora, case)
+    if name.endswith("_sa"):
+        gpu.init_uniform(case.rho, case.velocity, case.p)
+        turb = cases.turbulence(case, update=cfdg.TURB_FROZEN)
+        gpu.set_turbulence(turb)
+        ora.set_turbulence(turb)
     for s in (gpu, ora):
         s.set_state(U)
         s.eval_residual()

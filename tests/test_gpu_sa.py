@@ -111,12 +111,13 @@ def test_sa_implicit_steps_match_oracle(update):
     assert np.all(rg >= 0.0)
 
 
-def test_sa_steady_state_matches_oracle():
-    """Plate 16x8x4, Re_L 5e6, unlimited MUSCL, CFL 1000, point-implicit SA,
-    from 2 % noise: both sides run to a relative residual of 1e-11; primitive
-    fields within 1e-8 of their freestream scales, rho nu_tilde within 1e-8 of
-    its freestream value, wall force within 1e-8 relative."""
-    case = cases.plate(ni=16, nj=8, nk=4, re=5.0e6, limiter="UNLIMITED", cfl=1000.0)
+def _sa_plate_case():
+    return cases.plate(ni=16, nj=8, nk=4, re=5.0e6, limiter="UNLIMITED", cfl=1000.0)
+
+
+@pytest.fixture(scope="module")
+def sa_oracle_converged():
+    case = _sa_plate_case()
     mesh = cfdg.Mesh.generate(**case.mesh)
     o = Oracle(mesh, case.numerics, case.bcs)
     U0 = perturbed_state(o, case)
@@ -133,10 +134,29 @@ def test_sa_steady_state_matches_oracle():
     else:
         pytest.fail("oracle did not converge")
     o.eval_residual()
+    return mesh, U0, turb, o
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16"])
+def test_sa_steady_state_matches_oracle(sa_oracle_converged, form):
+    """Plate 16x8x4, Re_L 5e6, unlimited MUSCL, CFL 1000, point-implicit SA,
+    from 2 % noise: the FP64 oracle and the device in each precision form
+    (FP64; FP32 blocks + FP32 Krylov; BF16 blocks + FP32 Krylov, both inside
+    the FP64 defect correction) run to a relative residual of 1e-11; per
+    field, rho, u, v, w, p within 1e-8 of their freestream scales, rho
+    nu_tilde within 1e-8 of its freestream value, wall force within 1e-8
+    relative."""
+    case = _sa_plate_case()
+    mesh, U0, turb, o = sa_oracle_converged
+    lp = cases.default_linparams()
     s = cfdg.Solver(mesh, case.numerics, case.bcs)
     s.init_uniform(case.rho, case.velocity, case.p)
     s.set_turbulence(turb)
     s.set_state(U0)
+    if form == "fp32":
+        s.set_precision(cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1))
+    elif form == "bf16":
+        s.set_precision(cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=16, inner_rel_tol=0.1))
     r = s.run_steady(lp, 8000, rel_tol=1e-11)
     assert r.status == cfdg.CONVERGED, (r.status, r.steps, r.rel)
     s.eval_residual()
@@ -149,7 +169,7 @@ def test_sa_steady_state_matches_oracle():
     speed = float(np.linalg.norm(case.velocity))
     scale = np.array([case.rho, speed, speed, speed, case.p])[:, None]
     err = np.max(np.abs(prim(s.get_state()) - prim(o.get_state())) / scale, axis=1)
-    assert np.all(err <= 1e-8), err
+    assert np.all(err <= 1e-8), dict(zip(("rho", "u", "v", "w", "p"), err))
     tg, to = s.turbulence(), o.turbulence()
     rnu_inf = to["rho_nu"].max()
     assert np.abs(tg["rho_nu"] - to["rho_nu"]).max() <= 1e-8 * rnu_inf
