@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcfdg.so")
 
 # --- constants (include/cfdg.h) ---------------------------------------------
 OK = 0
-ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH, ERR_CAPACITY = -1, -2, -3, -4, -5, -6, -7
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_SINGULAR, ERR_CONFIG, ERR_MESH, ERR_CAPACITY, ERR_IO = -1, -2, -3, -4, -5, -6, -7, -8
 NOALLOC_MAX_CELLS = 1 << 30
 GRAD_GATHER, GRAD_LDS, GRAD_LDS_WIDE = 0, 1, 2
 INVISCID_FLOW, VISCOUS_FLOW = 0, 1
@@ -181,6 +181,17 @@ def _load():
         "cfdg_get_turbulence": [vp, vp, vp, vp, vp, vp, vp, vp, vp],
         "cfdg_residual_norms_noalloc": [vp, _i64, _i32, vp, vp],
         "cfdg_gradient_form": [vp, vp, vp, vp],
+        "cfdg_mesh_patch_cgns_type": [vp, _i32, C.c_char_p, _i32],
+        "cfdg_mesh_set_patch": [vp, _i32, C.c_char_p, C.c_char_p],
+        # host front end (include/cfdg_front.h)
+        "cfdg_config_read_solver": [C.c_char_p, vp, C.c_char_p, _i32],
+        "cfdg_config_read_solver_text": [C.c_char_p, C.c_char_p, vp, C.c_char_p, _i32],
+        "cfdg_config_read_boundary": [C.c_char_p, vp, vp, _i32, C.c_char_p, _i32, C.c_char_p, _i32],
+        "cfdg_config_read_boundary_text": [C.c_char_p, C.c_char_p, vp, _i32, vp, C.c_char_p, _i32, C.c_char_p, _i32],
+        "cfdg_config_to_numerics": [vp, vp, vp, C.c_char_p, _i32],
+        "cfdg_container_read": [C.c_char_p, vp, C.c_char_p, _i32],
+        "cfdg_container_write": [C.c_char_p, vp, C.c_char_p, _i32],
+        "cfdg_run_solver": [vp, vp, _i32, vp, _i32, C.c_char_p, _i32],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -294,6 +305,20 @@ class Mesh:
             lib.cfdg_mesh_patch_name(self._h, p, buf, 128)
             out.append(buf.value.decode())
         return out
+
+    def patch_cgns_types(self):
+        out = []
+        buf = C.create_string_buffer(128)
+        for p in range(self.n_patches):
+            lib.cfdg_mesh_patch_cgns_type(self._h, p, buf, 128)
+            out.append(buf.value.decode())
+        return out
+
+    def set_patch(self, patch, name=None, cgns_type=None):
+        rc = lib.cfdg_mesh_set_patch(self._h, patch, name.encode() if name is not None else None,
+                                     cgns_type.encode() if cgns_type is not None else None)
+        if rc != OK:
+            raise CfdgError(rc, f"set_patch {patch}")
 
     def cell_gid(self):
         """Index of each cell at generation / copy, carried through renumbering."""

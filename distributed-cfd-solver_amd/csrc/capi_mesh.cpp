@@ -51,6 +51,7 @@ cfdg::Mesh mesh_from_view(const cfdg_mesh_view& v) {
     m.face_nz.assign(v.face_nz, v.face_nz + nf);
     m.face_area.assign(v.face_area, v.face_area + nf);
     for (std::int32_t p = 0; p < v.n_patches; ++p) m.patch_names.push_back("patch" + std::to_string(p));
+    m.patch_cgns_types.assign(static_cast<std::size_t>(v.n_patches), "UserDefined");
     for (std::size_t i = 0; i < nc; ++i) {
         if (m.cell_nodes_off[i + 1] < m.cell_nodes_off[i]) throw CfdgError(CFDG_ERR_MESH, "cell node offsets");
     }
@@ -166,6 +167,22 @@ int cfdg_mesh_patch_name(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_
     if (!mesh || !buf || buflen <= 0) return CFDG_ERR_ARG;
     if (patch < 0 || patch >= static_cast<int32_t>(mesh->m.patch_names.size())) return CFDG_ERR_ARG;
     std::snprintf(buf, static_cast<std::size_t>(buflen), "%s", mesh->m.patch_names[static_cast<std::size_t>(patch)].c_str());
+    return CFDG_OK;
+}
+
+int cfdg_mesh_patch_cgns_type(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_t buflen) {
+    if (!mesh || !buf || buflen <= 0) return CFDG_ERR_ARG;
+    if (patch < 0 || patch >= static_cast<int32_t>(mesh->m.patch_cgns_types.size())) return CFDG_ERR_ARG;
+    std::snprintf(buf, static_cast<std::size_t>(buflen), "%s",
+                  mesh->m.patch_cgns_types[static_cast<std::size_t>(patch)].c_str());
+    return CFDG_OK;
+}
+
+int cfdg_mesh_set_patch(cfdg_mesh* mesh, int32_t patch, const char* name, const char* cgns_type) {
+    if (!mesh) return CFDG_ERR_ARG;
+    if (patch < 0 || patch >= static_cast<int32_t>(mesh->m.patch_names.size())) return CFDG_ERR_ARG;
+    if (name) mesh->m.patch_names[static_cast<std::size_t>(patch)] = name;
+    if (cgns_type) mesh->m.patch_cgns_types[static_cast<std::size_t>(patch)] = cgns_type;
     return CFDG_OK;
 }
 

@@ -440,6 +440,7 @@ Mesh generate_mesh(const MeshSpec& s) {
     m.face_nodes_off[nf] = static_cast<std::int32_t>(nf) * fnn;
     m.n_internal_faces = m.n_inner_faces;
     m.patch_names = names;
+    m.patch_cgns_types.assign(names.size(), "UserDefined");
 
     compute_geometry(m);
 

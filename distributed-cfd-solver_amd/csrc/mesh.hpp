@@ -45,8 +45,9 @@ struct Mesh {
     std::vector<double> face_cx, face_cy, face_cz;
     std::vector<double> face_nx, face_ny, face_nz, face_area;
 
-    // patches (names only; faces of patch p form one contiguous boundary range)
+    // patches (faces of patch p form one contiguous boundary range)
     std::vector<std::string> patch_names;
+    std::vector<std::string> patch_cgns_types;  // CGNS BC type names of the container ("UserDefined" by default)
 
     std::int32_t n_bfaces() const noexcept { return n_faces - n_inner_faces; }
 };

@@ -152,6 +152,12 @@ int cfdg_mesh_generate(const cfdg_mesh_spec* spec, cfdg_mesh** out);
 int cfdg_mesh_from_view(const cfdg_mesh_view* view, cfdg_mesh** out);
 int cfdg_mesh_get_view(const cfdg_mesh* mesh, cfdg_mesh_view* view);
 int cfdg_mesh_patch_name(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_t buflen);
+/* Patch metadata of MeshPart::patches (localmesh.hpp:86-92): the name and the
+   CGNS BC type string; meshes from cfdg_mesh_from_view start as "patch<p>" /
+   "UserDefined", generated ones as the generator's names / "UserDefined".
+   Either string may be NULL to keep it. */
+int cfdg_mesh_patch_cgns_type(const cfdg_mesh* mesh, int32_t patch, char* buf, int32_t buflen);
+int cfdg_mesh_set_patch(cfdg_mesh* mesh, int32_t patch, const char* name, const char* cgns_type);
 /* Global id of each cell (n_cells values; MeshPart::cell_gid,
    localmesh.hpp:17-102): the cell's index at generation or copy, carried
    through renumbering, so gid[new] = old index after cfdg_mesh_reorder_rcm

@@ -52,3 +52,17 @@ for p in "${pids[@]}"; do wait "$p"; done
 g++ -o "$OUT/ref_harness" "$OUT/obj/ref_harness.o" $OBJS -static-libstdc++ -static-libgcc \
     /opt/conda/lib/libmpi.so -Wl,-rpath,/opt/conda/lib
 echo "built $OUT/ref_harness"
+# the reference's configuration parsers (toml++ 3.4.0 is vendored under external/)
+CFG_SRCS="src/io/toml/toml_utilities.cpp src/solver/compressible/config.cpp src/bc/config.cpp"
+CFG_OBJS=""
+for s in $CFG_SRCS; do
+    o="$OUT/obj/$(echo "$s" | tr '/' '_' | sed 's/\.cpp$/.o/')"
+    CFG_OBJS="$CFG_OBJS $o"
+    if [ ! -f "$o" ] || [ "$REF/$s" -nt "$o" ]; then
+        "$MPICXX" $FLAGS -c "$REF/$s" -o "$o"
+    fi
+done
+"$MPICXX" $FLAGS -c "$HERE/ref_config.cpp" -o "$OUT/obj/ref_config.o"
+g++ -o "$OUT/ref_config" "$OUT/obj/ref_config.o" $CFG_OBJS "$OUT/obj/src_mpi_log.o" -static-libstdc++ -static-libgcc \
+    /opt/conda/lib/libmpi.so -Wl,-rpath,/opt/conda/lib
+echo "built $OUT/ref_config"

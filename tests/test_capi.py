@@ -1,5 +1,7 @@
 """The C-ABI library loads without a GPU and exports every entry point that
-include/cfdg.h declares; host-only entry points (mesh) work without a GPU."""
+include/*.h (cfdg.h, cfdg_front.h) declares; host-only entry points (mesh)
+work without a GPU."""
+import glob
 import ctypes as C
 import os
 import re
@@ -12,9 +14,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "cfdg.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(cfdg_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(cfdg_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_every_declared_symbol_is_exported():
