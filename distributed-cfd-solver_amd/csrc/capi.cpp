@@ -150,6 +150,17 @@ void upload_groups(cfdg_ctx& c, int slot, const std::vector<i32>& label_mesh, i3
     c.grows[slot].upload(rows, c.stream);
 }
 
+// Forward-sweep flags of every block (device order) from the rows' group
+// labels (mesh order): 1 when the column's group precedes the row's.
+void upload_lower_flags(cfdg_ctx& c, int slot, const std::vector<i32>& label_mesh) {
+    std::vector<std::uint8_t> f(c.h_cols.size(), 0);
+    for (i32 o = 0; o < c.nc; ++o)
+        for (i32 k = c.h_row_ptr[z(o)]; k < c.h_row_ptr[z(o) + 1]; ++k)
+            f[z(c.blk_dev[z(k)])] = label_mesh[z(c.h_cols[z(k)])] < label_mesh[z(o)] ? 1 : 0;
+    c.blk_lower[slot].upload(f, c.stream);
+    c.sync();  // f is released on return
+}
+
 void build_seq_groups(cfdg_ctx& c) {
     if (c.have_seq_groups) return;
     i32 nfw = 0, nbw = 0;
@@ -157,6 +168,7 @@ void build_seq_groups(cfdg_ctx& c) {
     const auto lb = cfdg::backward_levels(c.ccf_mesh, c.nc, nbw);
     upload_groups(c, 2, lf, nfw, false);
     upload_groups(c, 3, lb, nbw, false);
+    upload_lower_flags(c, 1, lf);
     c.sync();
     c.have_seq_groups = true;
 }
@@ -270,28 +282,32 @@ void spmv_stored(cfdg_ctx& c, const T* x, T* y, const double* gate = nullptr) {
 }
 
 // z = M^-1 r: z = 0, forward groups, backward groups (preconditioners.cpp:171-194).
-// The backward sweep starts with the group the forward sweep ended with: those
-// rows read only neighbours that the forward sweep already left final, so their
-// backward update equals their forward one bit for bit and is skipped.
-// z_zeroed: the caller's preceding kernel already set z to 0.
+// The forward sweep reads only the blocks whose column lies in an earlier
+// group -- the rows of later groups still hold z = 0 there, so the other
+// products add nothing (with the dependency levels these are exactly the
+// columns j < i of the reference's sweep) -- and it writes every row, so z
+// needs no clearing.  The backward sweep starts with the group the forward
+// sweep ended with: those rows read only neighbours that the forward sweep
+// already left final, so their backward update equals their forward one bit
+// for bit and is skipped.
 template <class T>
-void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr,
-                   bool z_zeroed = false) {
+void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr) {
     const long long n5 = 5LL * c.nc;
     if (P.kind == CFDG_PRECOND_NONE) {
         hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(T), hipMemcpyDeviceToDevice, c.stream),
                   "copy");
         return;
     }
-    if (!z_zeroed) hip_check(hipMemsetAsync(z, 0, static_cast<std::size_t>(n5) * sizeof(T), c.stream), "memset");
-    const int gf = (P.order == CFDG_SGS_SEQUENTIAL) ? 2 : 0;
+    const bool seq = P.order == CFDG_SGS_SEQUENTIAL;
+    const int gf = seq ? 2 : 0;
     const auto b = c.bargs();
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int gi = gf + sweep;
         const auto& off = c.h_gofs[gi];
+        const std::uint8_t* lower = sweep == 0 ? c.blk_lower[seq ? 1 : 0].get() : nullptr;
         for (std::size_t k = (sweep == 1 ? 1 : 0); k + 1 < off.size(); ++k)
             cfdg::launch_sgs_group<T>(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.storage,
-                                      c.lu.get(), c.piv.get(), r, z, c.stream, gate);
+                                      c.lu.get(), c.piv.get(), lower, r, z, c.stream, gate);
     }
 }
 
@@ -365,9 +381,8 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
             for (int b = 0; b < batch && enq <= lp.max_iter; ++b) {
                 ++enq;
                 cfdg::launch_bicg_begin(ks, c.stream);
-                const bool sgs = P.kind == CFDG_PRECOND_SGS;
-                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, sgs ? w.ph : nullptr, c.stream);
-                precond_apply<T>(c, P, w.p, w.ph, ks, sgs);
+                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, nullptr, c.stream);
+                precond_apply<T>(c, P, w.p, w.ph, ks);
                 spmv_stored<T>(c, w.ph, w.v, ks);
                 {
                     cfdg::DotPairsT<T> pr{};
@@ -376,8 +391,8 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                     cfdg::launch_dots(N, 1, pr, c.partial.get(), c.dots.get(), c.stream, ks);
                 }
                 cfdg::launch_bicg_alpha(ks, c.dots.get(), c.stream);
-                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, sgs ? w.sh : nullptr, c.stream);
-                precond_apply<T>(c, P, w.s, w.sh, ks, sgs);
+                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, nullptr, c.stream);
+                precond_apply<T>(c, P, w.s, w.sh, ks);
                 spmv_stored<T>(c, w.sh, w.t, ks);
                 {
                     cfdg::DotPairsT<T> pr{};
@@ -823,6 +838,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         const auto col = cfdg::greedy_colours(c.ccf_mesh, c.nc, ncol);
         upload_groups(c, 0, col, ncol, false);
         upload_groups(c, 1, col, ncol, true);
+        upload_lower_flags(c, 0, col);
         c.sync();
     }
     const std::size_t ncs = z(c.nc);

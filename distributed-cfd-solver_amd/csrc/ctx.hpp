@@ -108,6 +108,9 @@ struct cfdg_ctx {
     std::vector<std::int32_t> h_row_ptr, h_cols, h_diag;
     std::vector<long long> blk_dev;  // mesh block k -> device block
     DBuf<int> row_ptr, cols, diag, cf_blk;
+    // per block: 1 when its column lies in an earlier group of the forward SGS
+    // sweep than its row ([0] multicolour groups, [1] dependency levels)
+    DBuf<std::uint8_t> blk_lower[2];
     // block values and diagonal LU as raw bytes in the storage of the precision form
     DBuf<std::uint8_t> values, lu;
     int storage = cfdg::kStoreF64;

@@ -158,7 +158,8 @@ constexpr int kSgsRowsPerWave = 12;
 template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
                                                       const int* diag, const TS* values, const TL* lu,
-                                                      const std::uint8_t* piv, const TV* r, TV* z, const double* gate) {
+                                                      const std::uint8_t* piv, const std::uint8_t* lower, const TV* r,
+                                                      TV* z, const double* gate) {
     if (gate && gate[kKsStatus] != 0.0) return;
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int rw = lane / 5;  // row of the wave, 12 = idle lanes
@@ -186,7 +187,9 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
         const int d = diag[i];
         const int kb = row_ptr[i], ke = row_ptr[i + 1];
         for (int k = kb; k < ke; ++k) {
-            if (k == d) continue;
+            // lower: the sweep reads only the blocks whose column lies in an
+            // earlier group (the others multiply values not computed yet)
+            if (k == d || (lower && !lower[k])) continue;
             TV a[5];
             block_row(values, k, ii, a);
             const TV* zb = z + static_cast<long long>(cols[k]) * 5;
@@ -592,29 +595,34 @@ template void launch_spmv<float>(const BsrArgs&, const void*, int, const float*,
 
 template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
-                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate) {
+                      const std::uint8_t* piv, const std::uint8_t* lower, const TV* r, TV* z, hipStream_t s,
+                      const double* gate) {
     if (count <= 0) return;
     const dim3 g = grid_for((static_cast<long long>(count) + kSgsRowsPerWave - 1) / kSgsRowsPerWave * 64), blk(kBlock);
     if constexpr (sizeof(TV) == sizeof(double)) {
         if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
         hipLaunchKernelGGL((k_sgs_group<double, double, double>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
-                           static_cast<const double*>(values), static_cast<const double*>(lu), piv, r, z, gate);
+                           static_cast<const double*>(values), static_cast<const double*>(lu), piv, lower, r, z, gate);
     } else {
         if (storage == kStoreF32)
             hipLaunchKernelGGL((k_sgs_group<float, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
-                               b.diag, static_cast<const float*>(values), static_cast<const float*>(lu), piv, r, z, gate);
+                               b.diag, static_cast<const float*>(values), static_cast<const float*>(lu), piv, lower, r,
+                               z, gate);
         else if (storage == kStoreBF16)
             hipLaunchKernelGGL((k_sgs_group<bf16, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols, b.diag,
-                               static_cast<const bf16*>(values), static_cast<const float*>(lu), piv, r, z, gate);
+                               static_cast<const bf16*>(values), static_cast<const float*>(lu), piv, lower, r, z, gate);
         else
             hipLaunchKernelGGL((k_sgs_group<double, float, float>), g, blk, 0, s, rows, count, b.row_ptr, b.cols,
-                               b.diag, static_cast<const double*>(values), static_cast<const float*>(lu), piv, r, z, gate);
+                               b.diag, static_cast<const double*>(values), static_cast<const float*>(lu), piv, lower, r,
+                               z, gate);
     }
 }
 template void launch_sgs_group<double>(const int*, int, const BsrArgs&, const void*, int, const void*,
-                                       const std::uint8_t*, const double*, double*, hipStream_t, const double*);
+                                       const std::uint8_t*, const std::uint8_t*, const double*, double*, hipStream_t,
+                                       const double*);
 template void launch_sgs_group<float>(const int*, int, const BsrArgs&, const void*, int, const void*,
-                                      const std::uint8_t*, const float*, float*, hipStream_t, const double*);
+                                      const std::uint8_t*, const std::uint8_t*, const float*, float*, hipStream_t,
+                                      const double*);
 
 template <class T>
 void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s) {

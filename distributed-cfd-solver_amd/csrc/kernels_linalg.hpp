@@ -40,9 +40,12 @@ template <class TV>
 void launch_spmv(const BsrArgs& b, const void* values, int storage, const TV* x, TV* y, hipStream_t s,
                  const double* gate = nullptr);
 // TV = double: FP64 blocks and FP64 LU.  TV = float: any storage, FP32 LU.
+// lower (one flag per block, may be null = every block): only the blocks
+// flagged 1 -- columns in an earlier group of the sweep -- are read.
 template <class TV>
 void launch_sgs_group(const int* rows, int count, const BsrArgs& b, const void* values, int storage, const void* lu,
-                      const std::uint8_t* piv, const TV* r, TV* z, hipStream_t s, const double* gate = nullptr);
+                      const std::uint8_t* piv, const std::uint8_t* lower, const TV* r, TV* z, hipStream_t s,
+                      const double* gate = nullptr);
 template <class T>
 void launch_update_p(long long n, double beta, double nbo, const T* r, const T* v, T* p, hipStream_t s);
 template <class T>
