@@ -20,6 +20,7 @@ src/mpi/log.cpp
 src/mpi/mpi_util.cpp
 src/mesh/aux_connectivity.cpp
 src/mesh/aux_geometry.cpp
+src/mesh/geometry.cpp
 src/mesh/reorder.cpp
 src/mesh/validate.cpp
 src/numerics/gradient/gradient_operator.cpp

@@ -10,6 +10,7 @@
 #include <array>
 #include <cstdint>
 #include <cstdio>
+#include <limits>
 #include <cstring>
 #include <fstream>
 #include <iterator>
@@ -24,6 +25,7 @@
 #include "cfd/linalg/config.hpp"
 #include "cfd/mesh/aux_connectivity.hpp"
 #include "cfd/mesh/aux_geometry.hpp"
+#include "cfd/mesh/geometry.hpp"
 #include "cfd/mesh/localmesh.hpp"
 #include "cfd/mesh/reorder.hpp"
 #include "cfd/mesh/validate.hpp"
@@ -618,6 +620,28 @@ int main(int argc, char** argv) {
             w.put("rcm_face_patch", m2.face_patch);
             w.put("rcm_face_area", m2.face_area);
             w.put("rcm_cell_cx", m2.cell_centroid_x);
+        }
+        if (r.count("geometry") && scalar<std::int32_t>(r, "geometry") != 0) {
+            // reference cell and face metrics from the nodes and connectivity
+            // alone (geometry.cpp:301-539); the given metrics are discarded
+            mesh::MeshPart g = mp;
+            const double nan = std::numeric_limits<double>::quiet_NaN();
+            for (auto* v : {&g.cell_centroid_x, &g.cell_centroid_y, &g.cell_centroid_z, &g.cell_volume,
+                            &g.face_centroid_x, &g.face_centroid_y, &g.face_centroid_z, &g.face_normal_x,
+                            &g.face_normal_y, &g.face_normal_z, &g.face_area})
+                v->assign(v->size(), nan);
+            mesh::compute_mesh_geometry(g);
+            w.put("geo_cell_cx", g.cell_centroid_x);
+            w.put("geo_cell_cy", g.cell_centroid_y);
+            w.put("geo_cell_cz", g.cell_centroid_z);
+            w.put("geo_cell_vol", g.cell_volume);
+            w.put("geo_face_cx", g.face_centroid_x);
+            w.put("geo_face_cy", g.face_centroid_y);
+            w.put("geo_face_cz", g.face_centroid_z);
+            w.put("geo_face_nx", g.face_normal_x);
+            w.put("geo_face_ny", g.face_normal_y);
+            w.put("geo_face_nz", g.face_normal_z);
+            w.put("geo_face_area", g.face_area);
         }
         const auto bcfg = build_bcs(r);
         if (scalar<std::int32_t>(r, "flow_model") == 1) {

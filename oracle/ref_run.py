@@ -25,7 +25,7 @@ def available():
 
 
 def case_records(mesh_arrays, n_patches, numerics, bcs, U, init, linparams, steps=0, solver_path=1, rcm=0,
-                 validate=1):
+                 validate=1, geometry=0):
     """Input record dict for the harness from cfdg-style arguments."""
     rec = dict(mesh_arrays)
     rec["n_patches"] = np.array([n_patches], np.int32)
@@ -53,6 +53,7 @@ def case_records(mesh_arrays, n_patches, numerics, bcs, U, init, linparams, step
     rec["solver_path"] = np.array([solver_path], np.int32)
     rec["rcm"] = np.array([rcm], np.int32)
     rec["validate"] = np.array([validate], np.int32)
+    rec["geometry"] = np.array([geometry], np.int32)
     return rec
 
 

@@ -12,7 +12,7 @@ physical steps of dual time stepping (BDF1, BDF2) from the uniform state;
 the vtu_* fixtures the reference's solution files after a few implicit steps;
 plate_mut the residual and Jacobian with a prescribed eddy-viscosity field.
 
-usage: python tests/golden/make_golden.py [time|vtu|mut|rcm|linsys|<fixture name>]
+usage: python tests/golden/make_golden.py [time|vtu|mut|rcm|geometry|linsys|<fixture name>]
 """
 import os
 import sys
@@ -292,6 +292,41 @@ def make_rcm(name="rcm_plate", case=None):
     print(f"{name} written")
 
 
+GEOMETRY_MESHES = {
+    # stretched, jittered hexahedra: non-planar quadrilateral faces
+    "hex_jitter": dict(kind=0, ni=7, nj=5, nk=4, x0=-0.3, x1=1.7, y0=0.0, y1=0.8, z0=0.0, z1=0.6, y_ratio=1.3,
+                       jitter=0.2, seed=5, reorder=0),
+    # tetrahedra over a bump, renumbered
+    "tet_bump": dict(kind=1, ni=5, nj=4, nk=3, x0=0.0, x1=3.0, y0=0.0, y1=1.0, z0=0.0, z1=0.5, bump_h=0.08,
+                     bump_w=0.4, bump_x=1.5, jitter=0.1, seed=2, reorder=1),
+    # hexahedra with a split wall patch, renumbered
+    "hex_split_rcm": dict(kind=0, ni=8, nj=4, nk=2, x0=0.0, x1=2.0, y0=0.0, y1=0.5, z0=0.0, z1=0.1, y_ratio=1.6,
+                          split_ymin=1, split_ymin_x=0.5, reorder=1),
+}
+
+
+def make_geometry():
+    """Reference cell and face metrics of generated meshes, computed from
+    their nodes and connectivity by compute_mesh_geometry (geometry.cpp:301-539)."""
+    c = cases.ramp(ni=4, nj=2, nk=2)
+    fx = {}
+    for name, spec in GEOMETRY_MESHES.items():
+        mesh = cfdg.Mesh.generate(**spec)
+        arrays = mesh.arrays()
+        bcs = [c.bcs[0]] * mesh.n_patches
+        rec = ref_run.case_records(arrays, mesh.n_patches, c.numerics, bcs, np.ones((5, mesh.n_cells)),
+                                   [c.rho, *c.velocity, c.p], c.linparams, steps=0, solver_path=0, geometry=1)
+        out = ref_run.run(rec)
+        for k in ("cell_type", "cell_nodes_off", "cell_nodes", "node_x", "node_y", "node_z", "face_owner",
+                  "face_nodes_off", "face_nodes"):
+            fx[f"{name}_mesh_{k}"] = arrays[k]
+        for k in ("cell_cx", "cell_cy", "cell_cz", "cell_vol", "face_cx", "face_cy", "face_cz", "face_nx", "face_ny",
+                  "face_nz", "face_area"):
+            fx[f"{name}_ref_{k}"] = out[f"geo_{k}"]
+        print(f"geometry {name}: {mesh.n_cells} cells, {mesh.n_faces} faces")
+    np.savez_compressed(os.path.join(HERE, "geometry.npz"), **fx)
+
+
 def rcm_bump_case():
     """Tetrahedral bump in natural order: the face graph has cells of degree
     1 to 4, so the degree sort of the breadth-first search decides ties."""
@@ -310,6 +345,8 @@ if __name__ == "__main__":
     if only in ("", "rcm"):
         make_rcm()
         make_rcm("rcm_bump", rcm_bump_case())
+    if only in ("", "geometry"):
+        make_geometry()
     if only in ("", "time"):
         for args in time_configs():
             make_time(*args)
