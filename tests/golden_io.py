@@ -12,7 +12,7 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def fixture_names():
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz"))
-                  if not os.path.basename(p).startswith(("rcm_", "time_", "vtu_", "plate_mut", "linsys_")))
+                  if not os.path.basename(p).startswith(("rcm_", "time_", "vtu_", "plate_mut", "linsys_", "geometry")))
 
 
 def load(name):
