@@ -151,14 +151,18 @@ void upload_groups(cfdg_ctx& c, int slot, const std::vector<i32>& label_mesh, i3
 }
 
 // Forward-sweep flags of every block (device order) from the rows' group
-// labels (mesh order): 1 when the column's group precedes the row's.
+// labels (mesh order): 1 when the column's group precedes the row's; and of
+// every device row: 1 when it lies in the first group (label 0).
 void upload_lower_flags(cfdg_ctx& c, int slot, const std::vector<i32>& label_mesh) {
-    std::vector<std::uint8_t> f(c.h_cols.size(), 0);
-    for (i32 o = 0; o < c.nc; ++o)
+    std::vector<std::uint8_t> f(c.h_cols.size(), 0), first(z(c.nc), 0);
+    for (i32 o = 0; o < c.nc; ++o) {
+        first[z(c.old2new[z(o)])] = label_mesh[z(o)] == 0 ? 1 : 0;
         for (i32 k = c.h_row_ptr[z(o)]; k < c.h_row_ptr[z(o) + 1]; ++k)
             f[z(c.blk_dev[z(k)])] = label_mesh[z(c.h_cols[z(k)])] < label_mesh[z(o)] ? 1 : 0;
+    }
     c.blk_lower[slot].upload(f, c.stream);
-    c.sync();  // f is released on return
+    c.row_first[slot].upload(first, c.stream);
+    c.sync();  // f and first are released on return
 }
 
 void build_seq_groups(cfdg_ctx& c) {
@@ -289,9 +293,11 @@ void spmv_stored(cfdg_ctx& c, const T* x, T* y, const double* gate = nullptr) {
 // needs no clearing.  The backward sweep starts with the group the forward
 // sweep ended with: those rows read only neighbours that the forward sweep
 // already left final, so their backward update equals their forward one bit
-// for bit and is skipped.
+// for bit and is skipped.  first_done: the forward sweep's first group was
+// already solved by the kernel that formed r (vec_sgs_first).
 template <class T>
-void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr) {
+void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double* gate = nullptr,
+                   bool first_done = false) {
     const long long n5 = 5LL * c.nc;
     if (P.kind == CFDG_PRECOND_NONE) {
         hip_check(hipMemcpyAsync(z, r, static_cast<std::size_t>(n5) * sizeof(T), hipMemcpyDeviceToDevice, c.stream),
@@ -305,7 +311,7 @@ void precond_apply(cfdg_ctx& c, const Precond& P, const T* r, T* z, const double
         const int gi = gf + sweep;
         const auto& off = c.h_gofs[gi];
         const std::uint8_t* lower = sweep == 0 ? c.blk_lower[seq ? 1 : 0].get() : nullptr;
-        for (std::size_t k = (sweep == 1 ? 1 : 0); k + 1 < off.size(); ++k)
+        for (std::size_t k = (sweep == 1 || first_done ? 1 : 0); k + 1 < off.size(); ++k)
             cfdg::launch_sgs_group<T>(c.grows[gi].get() + off[k], off[k + 1] - off[k], b, c.values.get(), c.storage,
                                       c.lu.get(), c.piv.get(), lower, r, z, c.stream, gate);
     }
@@ -375,24 +381,35 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
         // batch; iterations past convergence or breakdown are no-ops.
         const int slot = (sizeof(T) == sizeof(double)) ? 0 : 1;
         int batch = std::max(1, c.last_iters[slot] - 1);
+        const bool sgs = P.kind == CFDG_PRECOND_SGS;
+        const std::uint8_t* first = c.row_first[P.order == CFDG_SGS_SEQUENTIAL ? 1 : 0].get();
         long long enq = 0;
         double* rr = c.dots.get() + 5;
         while (true) {
             for (int b = 0; b < batch && enq <= lp.max_iter; ++b) {
                 ++enq;
-                cfdg::launch_bicg_begin(ks, c.stream);
-                cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, nullptr, c.stream);
-                precond_apply<T>(c, P, w.p, w.ph, ks);
+                // p, and with SGS the first forward group of p^ = M^-1 p, in one launch
+                if (sgs) {
+                    cfdg::launch_vec_sgs_first<T>(0, c.nc, ks, w.r, w.v, w.p, first, c.storage, c.lu.get(),
+                                                  c.piv.get(), w.ph, c.stream);
+                } else {
+                    cfdg::launch_update_p_g<T>(N, ks, w.r, w.v, w.p, c.stream);
+                }
+                precond_apply<T>(c, P, w.p, w.ph, ks, sgs);
                 spmv_stored<T>(c, w.ph, w.v, ks);
                 {
                     cfdg::DotPairsT<T> pr{};
                     pr.a[0] = w.r0;
                     pr.b[0] = w.v;
-                    cfdg::launch_dots(N, 1, pr, c.partial.get(), c.dots.get(), c.stream, ks);
+                    cfdg::launch_dots_step(N, 1, pr, c.partial.get(), c.dots.get(), ks, cfdg::kStepAlpha, c.stream);
                 }
-                cfdg::launch_bicg_alpha(ks, c.dots.get(), c.stream);
-                cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, nullptr, c.stream);
-                precond_apply<T>(c, P, w.s, w.sh, ks);
+                if (sgs) {
+                    cfdg::launch_vec_sgs_first<T>(1, c.nc, ks, w.r, w.v, w.s, first, c.storage, c.lu.get(),
+                                                  c.piv.get(), w.sh, c.stream);
+                } else {
+                    cfdg::launch_axpy_alpha_g<T>(N, ks, w.r, w.v, w.s, c.stream);
+                }
+                precond_apply<T>(c, P, w.s, w.sh, ks, sgs);
                 spmv_stored<T>(c, w.sh, w.t, ks);
                 {
                     cfdg::DotPairsT<T> pr{};
@@ -402,18 +419,16 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                         pr.a[j] = A[j];
                         pr.b[j] = B[j];
                     }
-                    cfdg::launch_dots(N, 5, pr, c.partial.get(), c.dots.get(), c.stream, ks);
+                    cfdg::launch_dots_step(N, 5, pr, c.partial.get(), c.dots.get(), ks, cfdg::kStepOmega, c.stream);
                 }
-                cfdg::launch_bicg_omega(ks, c.dots.get(), c.stream);
                 cfdg::launch_update_xr_g<T>(N, ks, w.ph, w.sh, w.s, w.t, x, w.r, c.stream);
                 if (enq % 32 == 0) {  // true residual norm every 32 iterations
                     cfdg::DotPairsT<T> pr{};
                     pr.a[0] = w.r;
                     pr.b[0] = w.r;
-                    cfdg::launch_dots(N, 1, pr, c.partial.get(), rr, c.stream, ks);
-                    cfdg::launch_bicg_post(ks, rr, c.stream);
+                    cfdg::launch_dots_step(N, 1, pr, c.partial.get(), rr, ks, cfdg::kStepPost, c.stream);
                 } else {
-                    cfdg::launch_bicg_post(ks, nullptr, c.stream);
+                    cfdg::launch_bicg_post(ks, c.stream);
                 }
             }
             c.check_launch();

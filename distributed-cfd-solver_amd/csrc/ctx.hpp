@@ -111,6 +111,8 @@ struct cfdg_ctx {
     // per block: 1 when its column lies in an earlier group of the forward SGS
     // sweep than its row ([0] multicolour groups, [1] dependency levels)
     DBuf<std::uint8_t> blk_lower[2];
+    // per device row: 1 when it lies in the forward sweep's first group (same slots)
+    DBuf<std::uint8_t> row_first[2];
     // block values and diagonal LU as raw bytes in the storage of the precision form
     DBuf<std::uint8_t> values, lu;
     int storage = cfdg::kStoreF64;

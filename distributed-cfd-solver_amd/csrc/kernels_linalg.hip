@@ -155,6 +155,37 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int n, const int* row_ptr, cons
 // (lu_solve_block<5>, bsr_helpers.hpp:325-353) and keeps its component.  The
 // arithmetic and its order per component are those of one thread per row.
 constexpr int kSgsRowsPerWave = 12;
+
+// Component ii of the solution of (LU) x = s with partial pivoting
+// (lu_solve_block<5>, bsr_helpers.hpp:325-353): row swaps, forward then
+// backward substitution over the five values every lane of a row holds.
+template <class TV>
+__device__ __forceinline__ TV lu_solve_component(TV s[5], const TV Lr[25], const int pv[5], int ii) {
+    for (int k = 0; k < 5; ++k) {
+        const int pk = pv[k];
+        if (pk != k) {
+            TV tmp = s[k];
+            TV sp = s[0];
+            for (int q = 1; q < 5; ++q) sp = (q == pk) ? s[q] : sp;
+            s[k] = sp;
+            for (int q = 0; q < 5; ++q) s[q] = (q == pk) ? tmp : s[q];
+        }
+    }
+    for (int a = 1; a < 5; ++a) {
+        TV acc2 = s[a];
+        for (int b = 0; b < a; ++b) acc2 -= Lr[a * 5 + b] * s[b];
+        s[a] = acc2;
+    }
+    for (int a = 4; a >= 0; --a) {
+        TV acc2 = s[a];
+        for (int b = a + 1; b < 5; ++b) acc2 -= Lr[a * 5 + b] * s[b];
+        s[a] = acc2 / Lr[a * 5 + a];
+    }
+    TV out = s[0];
+    for (int q = 1; q < 5; ++q) out = (q == ii) ? s[q] : out;
+    return out;
+}
+
 template <class TS, class TL, class TV>
 __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count, const int* row_ptr, const int* cols,
                                                       const int* diag, const TS* values, const TL* lu,
@@ -210,29 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_group(const int* rows, int count
         for (int q = 0; q < 25; ++q) Lr[q] = load_as<TV>(L[q]);
         for (int q = 0; q < 5; ++q) pv[q] = p[q];
     }
-    for (int k = 0; k < 5; ++k) {
-        const int pk = pv[k];
-        if (pk != k) {
-            TV tmp = s[k];
-            TV sp = s[0];
-            for (int q = 1; q < 5; ++q) sp = (q == pk) ? s[q] : sp;
-            s[k] = sp;
-            for (int q = 0; q < 5; ++q) s[q] = (q == pk) ? tmp : s[q];
-        }
-    }
-    for (int a = 1; a < 5; ++a) {
-        TV acc2 = s[a];
-        for (int b = 0; b < a; ++b) acc2 -= Lr[a * 5 + b] * s[b];
-        s[a] = acc2;
-    }
-    for (int a = 4; a >= 0; --a) {
-        TV acc2 = s[a];
-        for (int b = a + 1; b < 5; ++b) acc2 -= Lr[a * 5 + b] * s[b];
-        s[a] = acc2 / Lr[a * 5 + a];
-    }
-    TV out = s[0];
-    for (int q = 1; q < 5; ++q) out = (q == ii) ? s[q] : out;
-    z[static_cast<long long>(i) * 5 + ii] = out;
+    z[static_cast<long long>(i) * 5 + ii] = lu_solve_component(s, Lr, pv, ii);
 }
 
 // p = ((p * beta) + (-beta * omega) v) + r           (bicgstab.cpp:104-109)
@@ -278,7 +287,30 @@ __global__ void __launch_bounds__(kBlock) k_update_xr(long long n, double alpha,
 // iterations enqueued after convergence or breakdown change nothing and the
 // result is the one the per-iteration host loop of bicgstab.cpp:42-201 gives.
 
-// Start of a solve (bicgstab.cpp:56-92): norms, tolerance, initial status.
+// Start of an iteration (bicgstab.cpp:97-109): iteration limit, it += 1,
+// beta = (rho / rho_old)(alpha / omega) from the second iteration on.  Run
+// at the end of the step before it (the start of the solve, or the end of the
+// previous iteration), since nothing in between reads or writes the state.
+__device__ void bicg_begin(double* ks) {
+    if (ks[kKsIt] >= ks[kKsMaxIt]) {
+        ks[kKsStatus] = kKsMaxIter;
+        return;
+    }
+    ks[kKsIt] += 1.0;
+    ks[kKsFirst] = (ks[kKsIt] == 1.0) ? 1.0 : 0.0;
+    if (ks[kKsFirst] == 0.0) {
+        const double beta = (ks[kKsRho] / ks[kKsRhoOld]) * (ks[kKsAlpha] / ks[kKsOmega]);
+        if (!isfinite(beta)) {
+            ks[kKsStatus] = kKsDiverged;
+            return;
+        }
+        ks[kKsBeta] = beta;
+        ks[kKsNbo] = -beta * ks[kKsOmega];
+    }
+}
+
+// Start of a solve (bicgstab.cpp:56-92): norms, tolerance, initial status,
+// then the start of the first iteration.
 __global__ void k_bicg_init(double* ks, const double* dots, int x_zero, int fallback, double abs_tol,
                             double rel_tol, int max_iter) {
     const double bnorm = sqrt(dots[0]);
@@ -304,6 +336,8 @@ __global__ void k_bicg_init(double* ks, const double* dots, int x_zero, int fall
         ks[kKsStatus] = kKsDiverged;
     else if (rnorm <= tol)
         ks[kKsStatus] = kKsConverged;
+    else
+        bicg_begin(ks);
 }
 
 // A rejected guess restarts from zero: x = 0, r = b.
@@ -316,30 +350,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_reject(long long n, const doubl
     r[i] = b[i];
 }
 
-// Start of an iteration (bicgstab.cpp:97-109): iteration limit, it += 1,
-// beta = (rho / rho_old)(alpha / omega) from the second iteration on.
-__global__ void k_bicg_begin(double* ks) {
-    if (ks[kKsStatus] != 0.0) return;
-    if (ks[kKsIt] >= ks[kKsMaxIt]) {
-        ks[kKsStatus] = kKsMaxIter;
-        return;
-    }
-    ks[kKsIt] += 1.0;
-    ks[kKsFirst] = (ks[kKsIt] == 1.0) ? 1.0 : 0.0;
-    if (ks[kKsFirst] == 0.0) {
-        const double beta = (ks[kKsRho] / ks[kKsRhoOld]) * (ks[kKsAlpha] / ks[kKsOmega]);
-        if (!isfinite(beta)) {
-            ks[kKsStatus] = kKsDiverged;
-            return;
-        }
-        ks[kKsBeta] = beta;
-        ks[kKsNbo] = -beta * ks[kKsOmega];
-    }
-}
-
 // gamma = (r0, v); alpha = rho / gamma (bicgstab.cpp:113-120)
-__global__ void k_bicg_alpha(double* ks, const double* dots) {
-    if (ks[kKsStatus] != 0.0) return;
+__device__ void bicg_alpha(double* ks, const double* dots) {
     const double gamma = dots[0];
     if (!isfinite(gamma) || gamma == 0.0) {
         ks[kKsStatus] = kKsDiverged;
@@ -350,8 +362,7 @@ __global__ void k_bicg_alpha(double* ks, const double* dots) {
 
 // omega = (t, s) / (t, t); the recurrence norm of the new r and rho_new
 // (bicgstab.cpp:126-160); dots = (t,t), (t,s), (r0,t), (r0,s), (s,s)
-__global__ void k_bicg_omega(double* ks, const double* dots) {
-    if (ks[kKsStatus] != 0.0) return;
+__device__ void bicg_omega(double* ks, const double* dots) {
     const double tt = dots[0], ts = dots[1], r0t = dots[2], r0s = dots[3], ss = dots[4];
     if (!isfinite(ts) || !isfinite(tt) || tt == 0.0) {
         ks[kKsStatus] = kKsDiverged;
@@ -365,9 +376,9 @@ __global__ void k_bicg_omega(double* ks, const double* dots) {
 }
 
 // After x and r are updated (bicgstab.cpp:160-195): breakdown checks, the
-// true residual norm when rr is given (every 32nd iteration), convergence.
-__global__ void k_bicg_post(double* ks, const double* rr) {
-    if (ks[kKsStatus] != 0.0) return;
+// true residual norm when rr is given (every 32nd iteration), convergence;
+// then the start of the next iteration.
+__device__ void bicg_post(double* ks, const double* rr) {
     double rnorm = ks[kKsRnNew];
     const double omega_new = ks[kKsOmegaNew];
     ks[kKsRnorm] = rnorm;
@@ -389,35 +400,83 @@ __global__ void k_bicg_post(double* ks, const double* rr) {
     ks[kKsRhoOld] = ks[kKsRho];
     ks[kKsRho] = rho_new;
     ks[kKsOmega] = omega_new;
+    bicg_begin(ks);
+}
+
+__global__ void k_bicg_post(double* ks) {
+    if (ks[kKsStatus] != 0.0) return;
+    bicg_post(ks, nullptr);
 }
 
 // p = r on the first iteration, else p = ((p beta) + (-beta omega) v) + r
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p,
-                                                       T* zero) {
-    if (ks[kKsStatus] != 0.0) return;
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (zero) zero[i] = static_cast<T>(0);  // the preconditioner's output, which its sweep starts from
-    if (ks[kKsFirst] != 0.0) {
-        p[i] = r[i];
-        return;
-    }
+__device__ __forceinline__ T p_entry(const double* ks, const T* r, const T* v, const T* p, long long i) {
+    if (ks[kKsFirst] != 0.0) return r[i];
     T t = p[i] * static_cast<T>(ks[kKsBeta]);
     t += static_cast<T>(ks[kKsNbo]) * v[i];
     t += static_cast<T>(1.0) * r[i];
-    p[i] = t;
+    return t;
 }
 
 // s = r + (-alpha) v
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv,
-                                                         T* zero) {
+__device__ __forceinline__ T s_entry(const double* ks, const T* r, const T* v, long long i) {
+    return r[i] + static_cast<T>(-ks[kKsAlpha]) * v[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p) {
     if (ks[kKsStatus] != 0.0) return;
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (zero) zero[i] = static_cast<T>(0);
-    sv[i] = r[i] + static_cast<T>(-ks[kKsAlpha]) * v[i];
+    p[i] = p_entry(ks, r, v, p, i);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v,
+                                                         T* sv) {
+    if (ks[kKsStatus] != 0.0) return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sv[i] = s_entry(ks, r, v, i);
+}
+
+// The SGS forward sweep's first group reads no off-diagonal block: its rows
+// solve D_i z_i = y_i with the diagonal factors alone.  These kernels form a
+// Krylov vector y -- p (kOp 0) or s (kOp 1), entry for entry as
+// k_update_p_g / k_axpy_alpha_g -- and for the rows flagged in `first` also
+// that solve, in k_sgs_group's lane layout and arithmetic, so z leaves them
+// as the sweep's first group would and the sweep starts at its second group.
+template <int kOp, class TL, class TV>
+__global__ void __launch_bounds__(kBlock) k_vec_sgs_first(int nrows, const double* ks, const TV* r, const TV* v,
+                                                          TV* y, const std::uint8_t* first, const TL* lu,
+                                                          const std::uint8_t* piv, TV* z) {
+    if (ks[kKsStatus] != 0.0) return;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int rw = lane / 5;
+    const int ii = lane - rw * 5;
+    const long long wave = (static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) >> 6;
+    const long long row = wave * kSgsRowsPerWave + rw;
+    const bool has_row = rw < kSgsRowsPerWave && row < nrows;  // uniform over the row's five lanes
+    const long long e = row * 5 + ii;
+    TV acc = 0;
+    bool solve = false;
+    if (has_row) {
+        acc = (kOp == 0) ? p_entry(ks, r, v, y, e) : s_entry(ks, r, v, e);
+        y[e] = acc;
+        solve = first[row] != 0;
+    }
+    const int base = rw * 5;
+    TV s[5];
+    for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, (base + q) & 63, 64);
+    if (!solve) return;
+    TV Lr[25];
+    int pv[5];
+    const TL* L = lu + row * 25;
+    const std::uint8_t* p = piv + row * 5;
+    for (int q = 0; q < 25; ++q) Lr[q] = load_as<TV>(L[q]);
+    for (int q = 0; q < 5; ++q) pv[q] = p[q];
+    z[e] = lu_solve_component(s, Lr, pv, ii);
 }
 
 // x = (x + alpha ph) + omega sh;  r = s + (-omega) t
@@ -520,6 +579,27 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(int nblocks, int nco
         s = block_sum(s, lds);
         if (threadIdx.x == 0) out[j] = s;
     }
+}
+
+// k_reduce_partials followed, in thread 0, by the scalar step of the
+// BiCGSTAB recurrence that consumes the sums (alpha, omega, or the end of an
+// iteration with the true residual norm), so the step needs no launch of its
+// own.  The block reads the state at its start, before thread 0 writes it.
+template <int kStep>
+__global__ void __launch_bounds__(kBlock) k_reduce_step(int nblocks, int ncomp, const double* partial, double* out,
+                                                        double* ks) {
+    if (ks[kKsStatus] != 0.0) return;
+    __shared__ double lds[kBlock / 64];
+    for (int j = 0; j < ncomp; ++j) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += kBlock) s += partial[b * 5 + j];
+        s = block_sum(s, lds);
+        if (threadIdx.x == 0) out[j] = s;
+    }
+    if (threadIdx.x != 0) return;
+    if constexpr (kStep == kStepAlpha) bicg_alpha(ks, out);
+    if constexpr (kStep == kStepOmega) bicg_omega(ks, out);
+    if constexpr (kStep == kStepPost) bicg_post(ks, out);
 }
 
 // The partial sums of k_res_sq added in k_reduce_partials' order, then
@@ -681,6 +761,23 @@ template void launch_dots<double>(long long, int, const DotPairsT<double>&, doub
 template void launch_dots<float>(long long, int, const DotPairsT<float>&, double*, double*, hipStream_t,
                                  const double*);
 
+template <class T>
+void launch_dots_step(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, double* ks,
+                      int step, hipStream_t s) {
+    const int nb = reduction_blocks(n);
+    hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kBlock), 0, s, n, ndot, pr, partial, ks);
+    if (step == kStepAlpha)
+        hipLaunchKernelGGL(k_reduce_step<kStepAlpha>, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out, ks);
+    else if (step == kStepOmega)
+        hipLaunchKernelGGL(k_reduce_step<kStepOmega>, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out, ks);
+    else
+        hipLaunchKernelGGL(k_reduce_step<kStepPost>, dim3(1), dim3(kBlock), 0, s, nb, ndot, partial, out, ks);
+}
+template void launch_dots_step<double>(long long, int, const DotPairsT<double>&, double*, double*, double*, int,
+                                       hipStream_t);
+template void launch_dots_step<float>(long long, int, const DotPairsT<float>&, double*, double*, double*, int,
+                                      hipStream_t);
+
 void launch_bicg_init(double* ks, const double* dots, int x_zero, int fallback, double abs_tol, double rel_tol,
                       int max_iter, hipStream_t s) {
     hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1), 0, s, ks, dots, x_zero, fallback, abs_tol, rel_tol, max_iter);
@@ -693,34 +790,49 @@ void launch_bicg_reject(long long n, const double* ks, const T* b, T* x, T* r, h
 template void launch_bicg_reject<double>(long long, const double*, const double*, double*, double*, hipStream_t);
 template void launch_bicg_reject<float>(long long, const double*, const float*, float*, float*, hipStream_t);
 
-void launch_bicg_begin(double* ks, hipStream_t s) { hipLaunchKernelGGL(k_bicg_begin, dim3(1), dim3(1), 0, s, ks); }
-void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1), 0, s, ks, dots);
-}
-void launch_bicg_omega(double* ks, const double* dots, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1), 0, s, ks, dots);
-}
-void launch_bicg_post(double* ks, const double* rr, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_post, dim3(1), dim3(1), 0, s, ks, rr);
-}
+void launch_bicg_post(double* ks, hipStream_t s) { hipLaunchKernelGGL(k_bicg_post, dim3(1), dim3(1), 0, s, ks); }
 
 template <class T>
-void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, T* zero, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_update_p_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, p, zero);
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_update_p_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, p);
 }
-template void launch_update_p_g<double>(long long, const double*, const double*, const double*, double*, double*,
-                                        hipStream_t);
-template void launch_update_p_g<float>(long long, const double*, const float*, const float*, float*, float*,
-                                       hipStream_t);
+template void launch_update_p_g<double>(long long, const double*, const double*, const double*, double*, hipStream_t);
+template void launch_update_p_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
 
-template <class T>
-void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, T* zero, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_axpy_alpha_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, sv, zero);
+template <class TV>
+void launch_vec_sgs_first(int op, int nrows, const double* ks, const TV* r, const TV* v, TV* y,
+                          const std::uint8_t* first, int storage, const void* lu, const std::uint8_t* piv, TV* z,
+                          hipStream_t s) {
+    if (nrows <= 0) return;
+    const dim3 g = grid_for((static_cast<long long>(nrows) + kSgsRowsPerWave - 1) / kSgsRowsPerWave * 64), blk(kBlock);
+    if constexpr (sizeof(TV) == sizeof(double)) {
+        if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
+        const double* L = static_cast<const double*>(lu);
+        if (op == 0)
+            hipLaunchKernelGGL((k_vec_sgs_first<0, double, double>), g, blk, 0, s, nrows, ks, r, v, y, first, L, piv, z);
+        else
+            hipLaunchKernelGGL((k_vec_sgs_first<1, double, double>), g, blk, 0, s, nrows, ks, r, v, y, first, L, piv, z);
+    } else {
+        const float* L = static_cast<const float*>(lu);  // FP32 factors with every FP32-vector storage
+        if (op == 0)
+            hipLaunchKernelGGL((k_vec_sgs_first<0, float, float>), g, blk, 0, s, nrows, ks, r, v, y, first, L, piv, z);
+        else
+            hipLaunchKernelGGL((k_vec_sgs_first<1, float, float>), g, blk, 0, s, nrows, ks, r, v, y, first, L, piv, z);
+    }
 }
-template void launch_axpy_alpha_g<double>(long long, const double*, const double*, const double*, double*, double*,
+template void launch_vec_sgs_first<double>(int, int, const double*, const double*, const double*, double*,
+                                           const std::uint8_t*, int, const void*, const std::uint8_t*, double*,
+                                           hipStream_t);
+template void launch_vec_sgs_first<float>(int, int, const double*, const float*, const float*, float*,
+                                          const std::uint8_t*, int, const void*, const std::uint8_t*, float*,
                                           hipStream_t);
-template void launch_axpy_alpha_g<float>(long long, const double*, const float*, const float*, float*, float*,
-                                         hipStream_t);
+
+template <class T>
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy_alpha_g<T>, grid_for(n), dim3(kBlock), 0, s, n, ks, r, v, sv);
+}
+template void launch_axpy_alpha_g<double>(long long, const double*, const double*, const double*, double*, hipStream_t);
+template void launch_axpy_alpha_g<float>(long long, const double*, const float*, const float*, float*, hipStream_t);
 
 template <class T>
 void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,

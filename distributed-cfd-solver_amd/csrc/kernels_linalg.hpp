@@ -61,7 +61,9 @@ int reduction_blocks(long long n);
 template <class T>
 void launch_dots(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, hipStream_t s,
                  const double* gate = nullptr);
-// device-driven BiCGSTAB: one-thread scalar steps and gated vector updates
+// device-driven BiCGSTAB: scalar steps on the state block ks and gated vector
+// updates.  The start of each iteration (iteration limit, beta) is taken at
+// the end of the step before it, so an iteration begins with its p update.
 // Start of a solve from dots = (b, b) and, unless x_zero, (r, r) with r = b - A x:
 // tolerance max(abs_tol, rel_tol ||b||), status, rho; with fallback, a guess
 // whose residual exceeds ||b|| is marked rejected (launch_bicg_reject then
@@ -70,15 +72,27 @@ void launch_bicg_init(double* ks, const double* dots, int x_zero, int fallback, 
                       int max_iter, hipStream_t s);
 template <class T>
 void launch_bicg_reject(long long n, const double* ks, const T* b, T* x, T* r, hipStream_t s);
-void launch_bicg_begin(double* ks, hipStream_t s);
-void launch_bicg_alpha(double* ks, const double* dots, hipStream_t s);
-void launch_bicg_omega(double* ks, const double* dots, hipStream_t s);
-void launch_bicg_post(double* ks, const double* rr, hipStream_t s);
+// dot products as launch_dots, gated by ks, then the scalar step that uses them
+// in the reducing kernel: kStepAlpha (dots = (r0, v)), kStepOmega (dots =
+// (t,t), (t,s), (r0,t), (r0,s), (s,s)) or kStepPost (dots = (r, r), the true
+// residual norm, and the end of the iteration)
+enum { kStepAlpha = 1, kStepOmega = 2, kStepPost = 3 };
 template <class T>
-// zero (may be null) is set to 0 alongside: the next preconditioner output
-void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, T* zero, hipStream_t s);
+void launch_dots_step(long long n, int ndot, const DotPairsT<T>& pr, double* partial, double* out, double* ks,
+                      int step, hipStream_t s);
+// the end of an iteration with the recurrence's residual norm
+void launch_bicg_post(double* ks, hipStream_t s);
 template <class T>
-void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, T* zero, hipStream_t s);
+void launch_update_p_g(long long n, const double* ks, const T* r, const T* v, T* p, hipStream_t s);
+template <class T>
+void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s);
+// y = p (op 0, as launch_update_p_g) or s (op 1, as launch_axpy_alpha_g) over
+// nrows block rows, and z_i = D_i^-1 y_i for the rows flagged in first (the SGS
+// forward sweep's first group, factors lu / piv as launch_sgs_group takes them)
+template <class TV>
+void launch_vec_sgs_first(int op, int nrows, const double* ks, const TV* r, const TV* v, TV* y,
+                          const std::uint8_t* first, int storage, const void* lu, const std::uint8_t* piv, TV* z,
+                          hipStream_t s);
 template <class T>
 void launch_update_xr_g(long long n, const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x, T* r,
                         hipStream_t s);
