@@ -350,7 +350,11 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
     const long long N = 5LL * c.nc;
     const std::size_t bytes = static_cast<std::size_t>(N) * sizeof(T);
     cfdg_iter_result res{CFDG_MAX_ITERATIONS, 0, 0.0, 0.0};
-    double* ks = c.ks.get();
+    // two state blocks: an iteration that ends in launch_xr_next reads one and
+    // leaves the next iteration's state in the other
+    double* const kb[2] = {c.ks.get(), c.ks.get() + cfdg::kKsCount};
+    int cur = 0;
+    double* ks = kb[0];
     double* h = c.h_ks;
     // r = b - A x (r = b for a zero guess), norms, tolerance and status on the
     // device: no host round trip before the first batch of iterations
@@ -385,11 +389,18 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
         const std::uint8_t* first = c.row_first[P.order == CFDG_SGS_SEQUENTIAL ? 1 : 0].get();
         long long enq = 0;
         double* rr = c.dots.get() + 5;
+        // with SGS, every iteration but the last of a batch (and the ones with
+        // the true residual norm) ends with launch_xr_next, which also forms
+        // the next iteration's p and the first group of its M^-1 p
+        bool p_ready = false;
         while (true) {
             for (int b = 0; b < batch && enq <= lp.max_iter; ++b) {
                 ++enq;
+                ks = kb[cur];
                 // p, and with SGS the first forward group of p^ = M^-1 p, in one launch
-                if (sgs) {
+                if (p_ready) {
+                    // formed by the previous iteration's launch_xr_next
+                } else if (sgs) {
                     cfdg::launch_vec_sgs_first<T>(0, c.nc, ks, w.r, w.v, w.p, first, c.storage, c.lu.get(),
                                                   c.piv.get(), w.ph, c.stream);
                 } else {
@@ -421,6 +432,15 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                     }
                     cfdg::launch_dots_step(N, 5, pr, c.partial.get(), c.dots.get(), ks, cfdg::kStepOmega, c.stream);
                 }
+                const bool next_in_batch = b + 1 < batch && enq <= lp.max_iter;
+                if (sgs && next_in_batch && enq % 32 != 0) {
+                    cfdg::launch_xr_next<T>(c.nc, ks, kb[1 - cur], w.ph, w.sh, w.s, w.t, x, w.r, w.v, w.p, first,
+                                            c.storage, c.lu.get(), c.piv.get(), c.stream);
+                    cur = 1 - cur;
+                    p_ready = true;
+                    continue;
+                }
+                p_ready = false;
                 cfdg::launch_update_xr_g<T>(N, ks, w.ph, w.sh, w.s, w.t, x, w.r, c.stream);
                 if (enq % 32 == 0) {  // true residual norm every 32 iterations
                     cfdg::DotPairsT<T> pr{};
@@ -432,6 +452,7 @@ cfdg_iter_result bicgstab(cfdg_ctx& c, const cfdg_linparams& lp, const Precond& 
                 }
             }
             c.check_launch();
+            ks = kb[cur];
             c.d2h(h, ks, cfdg::kKsCount);
             c.sync();
             if (h[cfdg::kKsStatus] != 0.0 || enq > lp.max_iter) break;
@@ -878,7 +899,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     }
     c.partial.alloc(static_cast<std::size_t>(cfdg::kReduceBlocks) * 5);
     c.dots.alloc(8);
-    c.ks.alloc(cfdg::kKsCount);
+    c.ks.alloc(2 * cfdg::kKsCount);
     c.sync();
 }
 

@@ -410,12 +410,32 @@ __global__ void k_bicg_post(double* ks) {
 
 // p = r on the first iteration, else p = ((p beta) + (-beta omega) v) + r
 template <class T>
-__device__ __forceinline__ T p_entry(const double* ks, const T* r, const T* v, const T* p, long long i) {
-    if (ks[kKsFirst] != 0.0) return r[i];
-    T t = p[i] * static_cast<T>(ks[kKsBeta]);
-    t += static_cast<T>(ks[kKsNbo]) * v[i];
-    t += static_cast<T>(1.0) * r[i];
+__device__ __forceinline__ T p_entry_of(double first, double beta, double nbo, T ri, const T* v, const T* p,
+                                        long long i) {
+    if (first != 0.0) return ri;
+    T t = p[i] * static_cast<T>(beta);
+    t += static_cast<T>(nbo) * v[i];
+    t += static_cast<T>(1.0) * ri;
     return t;
+}
+template <class T>
+__device__ __forceinline__ T p_entry(const double* ks, const T* r, const T* v, const T* p, long long i) {
+    return p_entry_of(ks[kKsFirst], ks[kKsBeta], ks[kKsNbo], r[i], v, p, i);
+}
+
+// x = (x + alpha ph) + omega sh, written; returns r = s + (-omega) t
+// (bicgstab.cpp:145-149)
+template <class T>
+__device__ __forceinline__ T xr_entry(const double* ks, const T* ph, const T* sh, const T* sv, const T* t, T* x,
+                                      long long i) {
+    const double alpha = ks[kKsAlpha], omega = ks[kKsOmegaNew];
+    T xv = x[i];
+    xv += static_cast<T>(alpha) * ph[i];
+    xv += static_cast<T>(omega) * sh[i];
+    x[i] = xv;
+    T rv = sv[i];
+    rv += static_cast<T>(-omega) * t[i];
+    return rv;
 }
 
 // s = r + (-alpha) v
@@ -480,20 +500,78 @@ __global__ void __launch_bounds__(kBlock) k_vec_sgs_first(int nrows, const doubl
 }
 
 // x = (x + alpha ph) + omega sh;  r = s + (-omega) t
+// The end of one iteration and the start of the next in one launch: x and r
+// as k_update_xr_g; the end-of-iteration checks and the next iteration's
+// start (bicg_post without the true residual, then bicg_begin) formed by
+// thread 0 of every block from the state ks_in alone (so every block takes
+// the same decision), block 0 writing the result to ks_out; then, unless the
+// solve has ended, p and the SGS forward sweep's first group of M^-1 p as
+// k_vec_sgs_first.  ph is read (x update) and written (z = M^-1 p, first group)
+// entry by entry by the same lane.
+template <class TL, class TV>
+__global__ void __launch_bounds__(kBlock) k_xr_next(int nrows, const double* ks_in, double* ks_out, TV* ph,
+                                                    const TV* sh, const TV* sv, const TV* t, TV* x, TV* r,
+                                                    const TV* v, TV* p, const std::uint8_t* first, const TL* lu,
+                                                    const std::uint8_t* piv) {
+    __shared__ double next[4];  // status, first, beta, nbo of the next iteration
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (ks_in[kKsStatus] != 0.0) {  // ended before: carry the state over
+        if (lead)
+            for (int k = 0; k < kKsCount; ++k) ks_out[k] = ks_in[k];
+        return;
+    }
+    if (threadIdx.x == 0) {
+        double st[kKsCount];
+        for (int k = 0; k < kKsCount; ++k) st[k] = ks_in[k];
+        bicg_post(st, nullptr);
+        next[0] = st[kKsStatus];
+        next[1] = st[kKsFirst];
+        next[2] = st[kKsBeta];
+        next[3] = st[kKsNbo];
+        if (lead)
+            for (int k = 0; k < kKsCount; ++k) ks_out[k] = st[k];
+    }
+    __syncthreads();
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int rw = lane / 5;
+    const int ii = lane - rw * 5;
+    const long long wave = (static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) >> 6;
+    const long long row = wave * kSgsRowsPerWave + rw;
+    const bool has_row = rw < kSgsRowsPerWave && row < nrows;  // uniform over the row's five lanes
+    const long long e = row * 5 + ii;
+    const bool go = next[0] == 0.0;  // uniform over the grid
+    TV acc = 0;
+    bool solve = false;
+    if (has_row) {
+        const TV rv = xr_entry(ks_in, ph, sh, sv, t, x, e);
+        r[e] = rv;
+        if (go) {
+            acc = p_entry_of(next[1], next[2], next[3], rv, v, p, e);
+            p[e] = acc;
+            solve = first[row] != 0;
+        }
+    }
+    if (!go) return;
+    const int base = rw * 5;
+    TV s[5];
+    for (int q = 0; q < 5; ++q) s[q] = __shfl(acc, (base + q) & 63, 64);
+    if (!solve) return;
+    TV Lr[25];
+    int pv[5];
+    const TL* L = lu + row * 25;
+    const std::uint8_t* pp = piv + row * 5;
+    for (int q = 0; q < 25; ++q) Lr[q] = load_as<TV>(L[q]);
+    for (int q = 0; q < 5; ++q) pv[q] = pp[q];
+    ph[e] = lu_solve_component(s, Lr, pv, ii);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_update_xr_g(long long n, const double* ks, const T* ph, const T* sh,
                                                         const T* sv, const T* t, T* x, T* r) {
     if (ks[kKsStatus] != 0.0) return;
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double alpha = ks[kKsAlpha], omega = ks[kKsOmegaNew];
-    T xv = x[i];
-    xv += static_cast<T>(alpha) * ph[i];
-    xv += static_cast<T>(omega) * sh[i];
-    x[i] = xv;
-    T rv = sv[i];
-    rv += static_cast<T>(-omega) * t[i];
-    r[i] = rv;
+    r[i] = xr_entry(ks, ph, sh, sv, t, x, i);
 }
 
 // dst = src converted (defect correction: FP64 residual -> FP32 inner right-hand side)
@@ -826,6 +904,28 @@ template void launch_vec_sgs_first<double>(int, int, const double*, const double
 template void launch_vec_sgs_first<float>(int, int, const double*, const float*, const float*, float*,
                                           const std::uint8_t*, int, const void*, const std::uint8_t*, float*,
                                           hipStream_t);
+
+template <class TV>
+void launch_xr_next(int nrows, const double* ks_in, double* ks_out, TV* ph, const TV* sh, const TV* sv, const TV* t,
+                    TV* x, TV* r, const TV* v, TV* p, const std::uint8_t* first, int storage, const void* lu,
+                    const std::uint8_t* piv, hipStream_t s) {
+    if (nrows <= 0) return;
+    const dim3 g = grid_for((static_cast<long long>(nrows) + kSgsRowsPerWave - 1) / kSgsRowsPerWave * 64), blk(kBlock);
+    if constexpr (sizeof(TV) == sizeof(double)) {
+        if (storage != kStoreF64) throw_bad_combination("FP64 SGS needs FP64 blocks");
+        hipLaunchKernelGGL((k_xr_next<double, double>), g, blk, 0, s, nrows, ks_in, ks_out, ph, sh, sv, t, x, r, v, p,
+                           first, static_cast<const double*>(lu), piv);
+    } else {
+        hipLaunchKernelGGL((k_xr_next<float, float>), g, blk, 0, s, nrows, ks_in, ks_out, ph, sh, sv, t, x, r, v, p,
+                           first, static_cast<const float*>(lu), piv);
+    }
+}
+template void launch_xr_next<double>(int, const double*, double*, double*, const double*, const double*,
+                                     const double*, double*, double*, const double*, double*, const std::uint8_t*,
+                                     int, const void*, const std::uint8_t*, hipStream_t);
+template void launch_xr_next<float>(int, const double*, double*, float*, const float*, const float*, const float*,
+                                    float*, float*, const float*, float*, const std::uint8_t*, int, const void*,
+                                    const std::uint8_t*, hipStream_t);
 
 template <class T>
 void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, T* sv, hipStream_t s) {

@@ -89,6 +89,14 @@ void launch_axpy_alpha_g(long long n, const double* ks, const T* r, const T* v, 
 // y = p (op 0, as launch_update_p_g) or s (op 1, as launch_axpy_alpha_g) over
 // nrows block rows, and z_i = D_i^-1 y_i for the rows flagged in first (the SGS
 // forward sweep's first group, factors lu / piv as launch_sgs_group takes them)
+// The end of an iteration (x, r, checks) and, unless the solve ends there,
+// the start of the next with p and the SGS forward sweep's first group of
+// ph = M^-1 p, in one launch; the state is read from ks_in and written to
+// ks_out (a second state block)
+template <class TV>
+void launch_xr_next(int nrows, const double* ks_in, double* ks_out, TV* ph, const TV* sh, const TV* sv, const TV* t,
+                    TV* x, TV* r, const TV* v, TV* p, const std::uint8_t* first, int storage, const void* lu,
+                    const std::uint8_t* piv, hipStream_t s);
 template <class TV>
 void launch_vec_sgs_first(int op, int nrows, const double* ks, const TV* r, const TV* v, TV* y,
                           const std::uint8_t* first, int storage, const void* lu, const std::uint8_t* piv, TV* z,
