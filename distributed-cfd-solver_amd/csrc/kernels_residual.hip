@@ -211,6 +211,10 @@ struct LdsScalar {
     }
 };
 
+template <class Self, class NbF, class FVec>
+__device__ __forceinline__ void store_and_limit(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
+                                                const Self& self, const double (&g)[5][3], NbF nbf, FVec fvec);
+
 // fvec(e, k, x0, y0, z0, ex, ey, ez) gives x_f - x_c of the cell's k-th face
 // (entry e of cf).
 template <class Self, class NbN, class NbF, class FVec, class Sc = NoScalar>
@@ -293,6 +297,18 @@ __device__ __forceinline__ void gradient_limiter_cell(const MeshArgs& m, const F
             g[v][2] *= iv;
         }
     }
+    store_and_limit(m, f, nm, c, self, g, nbf, fvec);
+}
+
+// The gradients g of device cell c stored, then the MUSCL limiter of the cell
+// from them (muscl.hpp:26-104): the second half of gradient_limiter_cell.
+template <class Self, class NbF, class FVec>
+__device__ __forceinline__ void store_and_limit(const MeshArgs& m, const FieldArgs& f, const NumArgs& nm, int c,
+                                                const Self& self, const double (&g)[5][3], NbF nbf, FVec fvec) {
+    const CellRefW rc = crecw(f, c);
+    double qc[5];
+    for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
+    const double x0 = self[kRC], y0 = self[kRC + 1], z0 = self[kRC + 2];
     for (int v = 0; v < 5; ++v)
         for (int d = 0; d < 3; ++d) rc[kRG + 3 * v + d] = g[v][d];
 
@@ -414,6 +430,123 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
         gradient_limiter_cell(m, f, nm, c0 + threadIdx.x, LdsSlot<kSlots>{sq, static_cast<int>(threadIdx.x)}, nbn, nbf,
                               fvec);
     }
+}
+
+// The wide staged form with two lanes per cell (workgroups of up to 512
+// lanes): its 1,088 LDS slots allow two workgroups per CU, so one lane per
+// cell would leave two waves per SIMD to hide the stencil loop's latencies.
+// The two lanes of a cell (adjacent lanes) split the cell's vertex stencil by
+// chunks of four entries (lane h takes chunks h, h + 2, ...), each adding
+// its entries in stencil order with the arithmetic of gradient_limiter_cell;
+// the second lane's sums are then added to the first lane's, which stores the
+// gradients and forms the limiter.  The sums are therefore those of the
+// one-lane form up to the order of that one addition.  Node LSQ only (the
+// only gradient the staged forms serve).
+template <bool kSA>
+__global__ void __launch_bounds__(2 * kTileCells) k_gradient_limiter_pair(MeshArgs m, FieldArgs f, NumArgs nm,
+                                                                          SaArgs sa) {
+    constexpr int kSlots = kGradSlotsBig;
+    __shared__ __attribute__((aligned(16))) double sq[(kSA ? 9 : 8) * kSlots];
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = m.gtile_off[t], nt = m.gtile_off[t + 1] - c0;
+    const int hb = m.tile_halo_off[t], ns = nt + (m.tile_halo_off[t + 1] - hb);
+    const int cell = static_cast<int>(threadIdx.x >> 1), half = static_cast<int>(threadIdx.x & 1);
+    const bool live = cell < nt;
+    const int nchn = (m.ccn_base[t + 1] - m.ccn_base[t]) / (4 * nt);
+    const uint2* chn = reinterpret_cast<const uint2*>(m.ccn_slot + m.ccn_base[t]) + cell;
+    // this lane's first two vertex-slot chunks, requested before the staging
+    uint2 cur = make_uint2(0u, 0u), nxt = cur;
+    if (live) {
+        if (half < nchn) cur = chn[half * nt];
+        if (half + 2 < nchn) nxt = chn[(half + 2) * nt];
+    }
+    for (int sl = threadIdx.x; sl < ns; sl += blockDim.x) {
+        const int src = (sl < nt) ? c0 + sl : m.tile_halo[hb + sl - nt];
+        for (int k = 0; k < 8; k += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(f.cs + plane_at(k, f.ld, src));
+            *reinterpret_cast<double2*>(sq + ((k >> 1) * kSlots + sl) * 2) = v;
+        }
+        if constexpr (kSA) sq[8 * kSlots + sl] = sa.nu[src];
+    }
+    __syncthreads();
+    const int c = c0 + cell;
+    const LdsSlot<kSlots> self{sq, cell};
+    double g[5][3], gs[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < 5; ++v) g[v][0] = g[v][1] = g[v][2] = 0.0;
+    if (live) {
+        double qc[5];
+        for (int v = 0; v < 5; ++v) qc[v] = self[kRQ + v];
+        const double x0 = self[kRC], y0 = self[kRC + 1], z0 = self[kRC + 2];
+        const double sc0 = kSA ? sq[8 * kSlots + cell] : 0.0;
+        const int cnt = m.ccn_off[c + 1] - m.ccn_off[c];
+        const CellRef ri{f.cs, f.ld, c};
+        const double i0 = ri[kRInv], i1 = ri[kRInv + 1], i2 = ri[kRInv + 2], i4 = ri[kRInv + 3], i5 = ri[kRInv + 4],
+                     i8 = ri[kRInv + 5];
+        for (int q = half; 4 * q < cnt; q += 2) {
+            const uint2 ch = cur;
+            cur = nxt;
+            if (q + 4 < nchn) nxt = chn[(q + 4) * nt];
+            for (int e = 0; e < 4 && 4 * q + e < cnt; ++e) {
+                const unsigned w = (e & 2) ? ch.y : ch.x;
+                const int slot = static_cast<int>((w >> ((e & 1) * 16)) & 0xffffu);
+                const LdsSlot<kSlots> rn{sq, slot};
+                const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double wgt = 1.0 / dmax(r2, 1.0e-30);
+                const double rx = wgt * dx, ry = wgt * dy, rz = wgt * dz;
+                const double cx = i0 * rx + i1 * ry + i2 * rz;
+                const double cy = i1 * rx + i4 * ry + i5 * rz;
+                const double cz = i2 * rx + i5 * ry + i8 * rz;
+                for (int v = 0; v < 5; ++v) {
+                    const double d = rn[kRQ + v] - qc[v];
+                    g[v][0] += cx * d;
+                    g[v][1] += cy * d;
+                    g[v][2] += cz * d;
+                }
+                if constexpr (kSA) {
+                    const double d = sq[8 * kSlots + slot] - sc0;
+                    gs[0] += cx * d;
+                    gs[1] += cy * d;
+                    gs[2] += cz * d;
+                }
+            }
+        }
+    }
+    // the second lane's sums added to the first lane's (every lane takes part)
+    for (int v = 0; v < 5; ++v)
+        for (int d = 0; d < 3; ++d) g[v][d] += __shfl_xor(g[v][d], 1, 64);
+    if constexpr (kSA)
+        for (int d = 0; d < 3; ++d) gs[d] += __shfl_xor(gs[d], 1, 64);
+    if (!live || half != 0) return;
+    if constexpr (kSA) {
+        const long long nc = m.nc;
+        sa.g[c] = gs[0];
+        sa.g[nc + c] = gs[1];
+        sa.g[2 * nc + c] = gs[2];
+    }
+    // the limiter's face neighbours and face vectors, in order, as the one-lane form reads them
+    const int nchf = (m.ccf_base[t + 1] - m.ccf_base[t]) / (4 * nt);
+    const uint2* chf = reinterpret_cast<const uint2*>(m.ccf_slot + m.ccf_base[t]) + cell;
+    uint2 curf = make_uint2(0u, 0u), nxtf = curf;
+    if (nchf > 0) curf = chf[0];
+    if (nchf > 1) nxtf = chf[nt];
+    const auto nbf = [&](int, int k) {
+        if ((k & 3) == 0 && k > 0) {
+            curf = nxtf;
+            if ((k >> 2) + 1 < nchf) nxtf = chf[((k >> 2) + 1) * nt];
+        }
+        const unsigned w = (k & 2) ? curf.y : curf.x;
+        return LdsSlot<kSlots>{sq, static_cast<int>((w >> ((k & 1) * 16)) & 0xffffu)};
+    };
+    const long long vb = m.face_vec_base[t];
+    const long long vw = (m.face_vec_base[t + 1] - vb) / (3LL * nt);
+    const double* fvp = m.face_vec + vb + cell;
+    const auto fvec = [&](int, int k, double, double, double, double& ex, double& ey, double& ez) {
+        ex = fvp[k * static_cast<long long>(nt)];
+        ey = fvp[(vw + k) * nt];
+        ez = fvp[(2 * vw + k) * nt];
+    };
+    store_and_limit(m, f, nm, c, self, g, nbf, fvec);
 }
 
 __device__ __forceinline__ void grad_dirichlet(double* g, const double* gi, double qin, double qfix, double nx,
@@ -1099,15 +1232,16 @@ bool launch_gradient_limiter(const MeshArgs& m, const FieldArgs& f, const NumArg
     if (m.tile_halo_off != nullptr) {
         const dim3 g(static_cast<unsigned>(m.ngtiles)), b(static_cast<unsigned>(m.gblock));
         const bool big = m.gslots > kGradSlots;
+        const dim3 b2(2u * static_cast<unsigned>(m.gblock));  // the wide form: two lanes per cell
         if (sa) {
             if (big)
-                hipLaunchKernelGGL((k_gradient_limiter_lds<true, kGradSlotsBig>), g, b, 0, s, m, f, nm, *sa);
+                hipLaunchKernelGGL((k_gradient_limiter_pair<true>), g, b2, 0, s, m, f, nm, *sa);
             else
                 hipLaunchKernelGGL((k_gradient_limiter_lds<true, kGradSlots>), g, b, 0, s, m, f, nm, *sa);
             return true;
         }
         if (big)
-            hipLaunchKernelGGL((k_gradient_limiter_lds<false, kGradSlotsBig>), g, b, 0, s, m, f, nm, SaArgs{});
+            hipLaunchKernelGGL((k_gradient_limiter_pair<false>), g, b2, 0, s, m, f, nm, SaArgs{});
         else
             hipLaunchKernelGGL((k_gradient_limiter_lds<false, kGradSlots>), g, b, 0, s, m, f, nm, SaArgs{});
         return false;
