@@ -152,7 +152,7 @@ def _load():
     vp = C.c_void_p
     sig = {
         "cfdg_mesh_generate": [vp, vp], "cfdg_mesh_from_view": [vp, vp], "cfdg_mesh_get_view": [vp, vp],
-        "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp],
+        "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp], "cfdg_mesh_geometry_device": [vp],
         "cfdg_mesh_cell_gid": [vp, vp],
         "cfdg_create": [vp, vp, vp, _i32, vp], "cfdg_destroy": [vp], "cfdg_sizes": [vp, vp, vp, vp],
         "cfdg_set_state": [vp, vp, _i64], "cfdg_get_state": [vp, vp, _i64], "cfdg_init_uniform": [vp, _d, vp, _d],
@@ -332,6 +332,14 @@ class Mesh:
         rc = lib.cfdg_mesh_reorder_rcm(self._h)
         if rc != OK:
             raise CfdgError(rc, "reorder failed")
+        lib.cfdg_mesh_get_view(self._h, C.byref(self.view))
+
+    def compute_geometry_on_device(self):
+        """Recomputes the cell and face metrics from the nodes on the device
+        (cfdg_mesh_geometry_device)."""
+        rc = lib.cfdg_mesh_geometry_device(self._h)
+        if rc != OK:
+            raise CfdgError(rc, "device geometry failed")
         lib.cfdg_mesh_get_view(self._h, C.byref(self.view))
 
 

@@ -4,8 +4,11 @@
 #include <stdexcept>
 #include <string>
 
+#include <hip/hip_runtime.h>
+
 #include "cfdg.h"
 #include "capi_internal.hpp"
+#include "kernels_mesh.hpp"
 #include "mesh.hpp"
 
 namespace {
@@ -60,6 +63,88 @@ cfdg::Mesh mesh_from_view(const cfdg_mesh_view& v) {
     const std::string e = cfdg::check_mesh(m);
     if (!e.empty()) throw CfdgError(CFDG_ERR_MESH, e);
     return m;
+}
+
+}  // namespace
+
+namespace {
+
+// Device buffer of one geometry computation, freed on scope exit.
+template <class T>
+struct DevArray {
+    T* p = nullptr;
+    explicit DevArray(std::size_t n) {
+        if (n > 0 && hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T)) != hipSuccess)
+            throw CfdgError(CFDG_ERR_HIP, "hipMalloc failed");
+    }
+    DevArray(const std::vector<T>& h) : DevArray(h.size()) {
+        if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            throw CfdgError(CFDG_ERR_HIP, "hipMemcpy to device failed");
+    }
+    ~DevArray() {
+        if (p) (void)hipFree(p);
+    }
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    void get(std::vector<T>& h) const {
+        if (!h.empty() && hipMemcpy(h.data(), p, h.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+            throw CfdgError(CFDG_ERR_HIP, "hipMemcpy to host failed");
+    }
+};
+
+void geometry_on_device(cfdg::Mesh& m) {
+    const auto nc = static_cast<std::size_t>(m.n_cells), nf = static_cast<std::size_t>(m.n_faces);
+    for (std::size_t c = 0; c < nc; ++c) {
+        const int n = m.cell_nodes_off[c + 1] - m.cell_nodes_off[c];
+        const bool tet = m.cell_type[c] == static_cast<std::uint8_t>(cfdg::CellKind::Tet);
+        const bool hex = m.cell_type[c] == static_cast<std::uint8_t>(cfdg::CellKind::Hexa);
+        if (!((tet && n == 4) || (hex && n == 8)))
+            throw CfdgError(CFDG_ERR_MESH, "cell " + std::to_string(c) + ": only tetrahedra and hexahedra");
+    }
+    for (std::size_t f = 0; f < nf; ++f) {
+        const int n = m.face_nodes_off[f + 1] - m.face_nodes_off[f];
+        if (n < 3 || n > 8) throw CfdgError(CFDG_ERR_MESH, "face " + std::to_string(f) + " has " + std::to_string(n) + " nodes");
+    }
+    const auto nn = static_cast<std::int64_t>(m.node_x.size());
+    if (m.node_y.size() != m.node_x.size() || m.node_z.size() != m.node_x.size() ||
+        m.cell_nodes.size() != static_cast<std::size_t>(m.cell_nodes_off[nc]) ||
+        m.face_nodes.size() != static_cast<std::size_t>(m.face_nodes_off[nf]))
+        throw CfdgError(CFDG_ERR_MESH, "node lists and offsets disagree");
+    for (const auto* list : {&m.cell_nodes, &m.face_nodes})
+        for (const int n : *list)
+            if (n < 0 || n >= nn) throw CfdgError(CFDG_ERR_MESH, "node index " + std::to_string(n) + " out of range");
+    DevArray<double> x(m.node_x), y(m.node_y), z(m.node_z);
+    DevArray<int> coff(m.cell_nodes_off), cn(m.cell_nodes), foff(m.face_nodes_off), fn(m.face_nodes);
+    DevArray<std::uint8_t> ct(m.cell_type);
+    DevArray<double> ccx(nc), ccy(nc), ccz(nc), cvol(nc), fcx(nf), fcy(nf), fcz(nf), fnx(nf), fny(nf), fnz(nf),
+        farea(nf);
+    const cfdg::GeomArgs a{m.n_cells, m.n_faces, x.p, y.p, z.p, coff.p, cn.p, ct.p, foff.p, fn.p,
+                           ccx.p, ccy.p, ccz.p, cvol.p, fcx.p, fcy.p, fcz.p, fnx.p, fny.p, fnz.p, farea.p};
+    cfdg::launch_mesh_geometry(a, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        throw CfdgError(CFDG_ERR_HIP, "geometry kernels failed");
+    std::vector<double> vol(nc), area(nf);
+    cvol.get(vol);
+    farea.get(area);
+    for (std::size_t c = 0; c < nc; ++c)
+        if (!(vol[c] > 1e-15))
+            throw CfdgError(CFDG_ERR_MESH, "degenerate cell " + std::to_string(c) + " volume " + std::to_string(vol[c]));
+    for (std::size_t f = 0; f < nf; ++f)
+        if (!(area[f] > 1e-15))
+            throw CfdgError(CFDG_ERR_MESH, "degenerate face " + std::to_string(f) + " area " + std::to_string(area[f]));
+    m.cell_vol = std::move(vol);
+    m.face_area = std::move(area);
+    for (auto* v : {&m.cell_cx, &m.cell_cy, &m.cell_cz}) v->resize(nc);
+    for (auto* v : {&m.face_cx, &m.face_cy, &m.face_cz, &m.face_nx, &m.face_ny, &m.face_nz}) v->resize(nf);
+    ccx.get(m.cell_cx);
+    ccy.get(m.cell_cy);
+    ccz.get(m.cell_cz);
+    fcx.get(m.face_cx);
+    fcy.get(m.face_cy);
+    fcz.get(m.face_cz);
+    fnx.get(m.face_nx);
+    fny.get(m.face_ny);
+    fnz.get(m.face_nz);
 }
 
 }  // namespace
@@ -199,6 +284,23 @@ int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh) {
         return CFDG_OK;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "cfdg_mesh_reorder_rcm: %s\n", e.what());
+        return CFDG_ERR_MESH;
+    }
+}
+
+
+int cfdg_mesh_geometry_device(cfdg_mesh* mesh) {
+    if (!mesh) return CFDG_ERR_ARG;
+    try {
+        cfdg::Mesh m = mesh->m;  // the mesh is left unchanged on error
+        geometry_on_device(m);
+        mesh->m = std::move(m);
+        return CFDG_OK;
+    } catch (const CfdgError& e) {
+        std::fprintf(stderr, "cfdg_mesh_geometry_device: %s\n", e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "cfdg_mesh_geometry_device: %s\n", e.what());
         return CFDG_ERR_MESH;
     }
 }

@@ -2,6 +2,8 @@
 // order for a single-rank mesh.  See mesh.hpp for the contract.
 #include "mesh.hpp"
 
+#include "geometry.hpp"
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -19,7 +21,8 @@ namespace {
 
 // Face tables of the CGNS volume elements used here: local node lists whose
 // right-hand-rule normal points out of a positively oriented element
-// (include/cfd/mesh/cgnstables.hpp:45-75).
+// (include/cfd/mesh/cgnstables.hpp:45-75); geometry.hpp holds the same
+// tables for the cell metrics.
 constexpr int kTetFaces[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
 constexpr int kHexFaces[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
                                  {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
@@ -36,70 +39,6 @@ inline std::uint64_t splitmix64(std::uint64_t x) noexcept {
 inline double uniform_pm1(std::uint64_t seed, std::uint64_t stream) noexcept {
     const std::uint64_t r = splitmix64(seed * 0xD1B54A32D192ED03ULL ^ splitmix64(stream));
     return static_cast<double>(r >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
-
-struct CellMetrics {
-    double vol, cx, cy, cz;
-};
-
-// Volume and centroid by decomposition into tetrahedra spanned by the vertex
-// average x0 and each (sub-)triangle of the faces (geometry.cpp:24-150).
-CellMetrics cell_metrics(int type, const double* x, const double* y, const double* z) {
-    const int nn = (type == 0) ? 4 : 8;
-    const int nf = (type == 0) ? 4 : 6;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int i = 0; i < nn; ++i) {
-        sx += x[i];
-        sy += y[i];
-        sz += z[i];
-    }
-    const double inv_n = 1.0 / static_cast<double>(nn);
-    const double x0 = sx * inv_n, y0 = sy * inv_n, z0 = sz * inv_n;
-
-    double vol = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
-    for (int f = 0; f < nf; ++f) {
-        if (type == 0) {
-            const int a = kTetFaces[f][0], b = kTetFaces[f][1], c = kTetFaces[f][2];
-            const double e1x = x[b] - x[a], e1y = y[b] - y[a], e1z = z[b] - z[a];
-            const double e2x = x[c] - x[a], e2y = y[c] - y[a], e2z = z[c] - z[a];
-            const double Sx = 0.5 * (e1y * e2z - e1z * e2y);
-            const double Sy = 0.5 * (e1z * e2x - e1x * e2z);
-            const double Sz = 0.5 * (e1x * e2y - e1y * e2x);
-            const double vt = (1.0 / 3.0) * (Sx * (x[a] - x0) + Sy * (y[a] - y0) + Sz * (z[a] - z0));
-            vol += vt;
-            mx += vt * (0.25 * (x0 + x[a] + x[b] + x[c]));
-            my += vt * (0.25 * (y0 + y[a] + y[b] + y[c]));
-            mz += vt * (0.25 * (z0 + z[a] + z[b] + z[c]));
-        } else {
-            double fx = 0.0, fy = 0.0, fz = 0.0;
-            for (int j = 0; j < 4; ++j) {
-                fx += x[kHexFaces[f][j]];
-                fy += y[kHexFaces[f][j]];
-                fz += z[kHexFaces[f][j]];
-            }
-            fx *= 0.25;
-            fy *= 0.25;
-            fz *= 0.25;
-            for (int j = 0; j < 4; ++j) {
-                const int a = kHexFaces[f][j], b = kHexFaces[f][(j + 1) & 3];
-                const double e1x = x[a] - fx, e1y = y[a] - fy, e1z = z[a] - fz;
-                const double e2x = x[b] - fx, e2y = y[b] - fy, e2z = z[b] - fz;
-                const double Sx = 0.5 * (e1y * e2z - e1z * e2y);
-                const double Sy = 0.5 * (e1z * e2x - e1x * e2z);
-                const double Sz = 0.5 * (e1x * e2y - e1y * e2x);
-                const double vt = (1.0 / 3.0) * (Sx * (fx - x0) + Sy * (fy - y0) + Sz * (fz - z0));
-                vol += vt;
-                mx += vt * (0.25 * (x0 + fx + x[a] + x[b]));
-                my += vt * (0.25 * (y0 + fy + y[a] + y[b]));
-                mz += vt * (0.25 * (z0 + fz + z[a] + z[b]));
-            }
-        }
-    }
-    if (vol > 1e-15) {
-        const double iv = 1.0 / vol;
-        return {vol, mx * iv, my * iv, mz * iv};
-    }
-    return {vol, x0, y0, z0};
 }
 
 struct HalfFace {
@@ -127,7 +66,7 @@ void compute_geometry(Mesh& m) {
             bz[k - b] = m.node_z[n];
         }
         const int type = (m.cell_type[c] == static_cast<std::uint8_t>(CellKind::Tet)) ? 0 : 3;
-        const CellMetrics cm = cell_metrics(type, bx, by, bz);
+        const geom::CellMetrics cm = geom::cell_metrics(type, bx, by, bz);
         if (!(cm.vol > 1e-15)) {
             std::ostringstream os;
             os << "degenerate cell " << c << " volume " << cm.vol;
@@ -149,71 +88,30 @@ void compute_geometry(Mesh& m) {
     for (std::size_t f = 0; f < nf; ++f) {
         const int b = m.face_nodes_off[f], e = m.face_nodes_off[f + 1];
         const int nn = e - b;
-        const auto node = [&](int k) { return static_cast<std::size_t>(m.face_nodes[static_cast<std::size_t>(b + k)]); };
-        double Sx = 0.0, Sy = 0.0, Sz = 0.0, fcx = 0.0, fcy = 0.0, fcz = 0.0;
-        if (nn == 3) {
-            const auto n0 = node(0), n1 = node(1), n2 = node(2);
-            const double e1x = m.node_x[n1] - m.node_x[n0], e1y = m.node_y[n1] - m.node_y[n0],
-                         e1z = m.node_z[n1] - m.node_z[n0];
-            const double e2x = m.node_x[n2] - m.node_x[n0], e2y = m.node_y[n2] - m.node_y[n0],
-                         e2z = m.node_z[n2] - m.node_z[n0];
-            Sx = 0.5 * (e1y * e2z - e1z * e2y);
-            Sy = 0.5 * (e1z * e2x - e1x * e2z);
-            Sz = 0.5 * (e1x * e2y - e1y * e2x);
-            fcx = (m.node_x[n0] + m.node_x[n1] + m.node_x[n2]) / 3.0;
-            fcy = (m.node_y[n0] + m.node_y[n1] + m.node_y[n2]) / 3.0;
-            fcz = (m.node_z[n0] + m.node_z[n1] + m.node_z[n2]) / 3.0;
-        } else {
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            for (int k = 0; k < nn; ++k) {
-                sx += m.node_x[node(k)];
-                sy += m.node_y[node(k)];
-                sz += m.node_z[node(k)];
-            }
-            const double inv = 1.0 / static_cast<double>(nn);
-            const double f0x = sx * inv, f0y = sy * inv, f0z = sz * inv;
-            double wx = 0.0, wy = 0.0, wz = 0.0, wa = 0.0;
-            for (int k = 0; k < nn; ++k) {
-                const auto p = node(k), q = node(k + 1 == nn ? 0 : k + 1);
-                const double e1x = m.node_x[p] - f0x, e1y = m.node_y[p] - f0y, e1z = m.node_z[p] - f0z;
-                const double e2x = m.node_x[q] - f0x, e2y = m.node_y[q] - f0y, e2z = m.node_z[q] - f0z;
-                const double tx = 0.5 * (e1y * e2z - e1z * e2y);
-                const double ty = 0.5 * (e1z * e2x - e1x * e2z);
-                const double tz = 0.5 * (e1x * e2y - e1y * e2x);
-                const double ta = std::sqrt(tx * tx + ty * ty + tz * tz);
-                Sx += tx;
-                Sy += ty;
-                Sz += tz;
-                wx += ta * ((f0x + m.node_x[p] + m.node_x[q]) / 3.0);
-                wy += ta * ((f0y + m.node_y[p] + m.node_y[q]) / 3.0);
-                wz += ta * ((f0z + m.node_z[p] + m.node_z[q]) / 3.0);
-                wa += ta;
-            }
-            if (wa > 1e-15) {
-                const double ia = 1.0 / wa;
-                fcx = wx * ia;
-                fcy = wy * ia;
-                fcz = wz * ia;
-            } else {
-                fcx = f0x;
-                fcy = f0y;
-                fcz = f0z;
-            }
-        }
-        const double area = std::sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
-        if (!(area > 1e-15)) {
+        if (nn < 3 || nn > 8) {
             std::ostringstream os;
-            os << "degenerate face " << f << " area " << area;
+            os << "face " << f << " has " << nn << " nodes";
             throw std::runtime_error(os.str());
         }
-        const double ia = 1.0 / area;
-        m.face_cx[f] = fcx;
-        m.face_cy[f] = fcy;
-        m.face_cz[f] = fcz;
-        m.face_nx[f] = Sx * ia;
-        m.face_ny[f] = Sy * ia;
-        m.face_nz[f] = Sz * ia;
-        m.face_area[f] = area;
+        for (int k = 0; k < nn; ++k) {
+            const auto n = static_cast<std::size_t>(m.face_nodes[static_cast<std::size_t>(b + k)]);
+            bx[k] = m.node_x[n];
+            by[k] = m.node_y[n];
+            bz[k] = m.node_z[n];
+        }
+        const geom::FaceMetrics fm = geom::face_metrics(nn, bx, by, bz);
+        if (!(fm.area > 1e-15)) {
+            std::ostringstream os;
+            os << "degenerate face " << f << " area " << fm.area;
+            throw std::runtime_error(os.str());
+        }
+        m.face_cx[f] = fm.cx;
+        m.face_cy[f] = fm.cy;
+        m.face_cz[f] = fm.cz;
+        m.face_nx[f] = fm.nx;
+        m.face_ny[f] = fm.ny;
+        m.face_nz[f] = fm.nz;
+        m.face_area[f] = fm.area;
     }
 }
 

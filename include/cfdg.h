@@ -166,6 +166,14 @@ int cfdg_mesh_cell_gid(const cfdg_mesh* mesh, int64_t* gid);
 /* Renumbers cells in reverse Cuthill-McKee order and re-sorts faces
    (src/mesh/reorder.cpp:135-188, 306-576). */
 int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh);
+/* Recomputes the cell centroids and volumes and the face centroids, unit
+   normals and areas from the nodes and node lists on the device
+   (compute_mesh_geometry, src/mesh/geometry.cpp:301-539), with the same
+   arithmetic as the host computation of cfdg_mesh_generate, so the values
+   are the same bit for bit.  Tetrahedra and hexahedra; faces of 3 to 8
+   nodes.  CFDG_ERR_MESH for a degenerate cell or face (volume or area not
+   above 1e-15), the mesh then left unchanged. */
+int cfdg_mesh_geometry_device(cfdg_mesh* mesh);
 void cfdg_mesh_free(cfdg_mesh* mesh);
 
 /* ---- solver context ----------------------------------------------------- */
