@@ -12,7 +12,8 @@ Prints ONE JSON line.  Extra fields: the flux kernel's roofline, the per-stage
 device times, implicit steps (Jacobian + BiCGSTAB/SGS) in the three precision
 forms, the CPU baseline (the reference's own compiled code where available,
 else the C++ oracle port), the time to a 1e-8 residual drop on a 262,144-cell
-plate in four forms (time_to_drop) and the 20.25M-tet transonic SA bump:
+plate in four forms (time_to_drop), the learned warm start on a 4.19M-cell
+plate timed to an equal drop (config4) and the 20.25M-tet transonic SA bump:
 its residual and the forms timed to an equal drop (config5).
 
 usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5]
@@ -45,6 +46,11 @@ def parse():
     ap.add_argument("--drop-previous-update", action="store_true",
                     help="also time the previous-update guess alone (the network with all parameters zero)")
     ap.add_argument("--drop-max-steps", type=int, default=12000)
+    ap.add_argument("--config4", default="256x64x256",
+                    help="plate mesh of the learned warm start at 4.19M cells ('off' skips it)")
+    ap.add_argument("--config4-cfl", type=float, default=100.0)
+    ap.add_argument("--config4-steps", type=int, default=150,
+                    help="implicit steps of the plain FP64 run that fix the drop the other forms are timed to")
     ap.add_argument("--config5", default="150",
                     help="hexahedra per direction of the transonic SA tetrahedral bump (each split into 6 Kuhn "
                          "tetrahedra; 150 = the 20.25M-tet case), or off")
@@ -215,6 +221,8 @@ def main():
     gnn = None
     if a.drop_size != "off":
         out["time_to_drop"], gnn = time_to_drop(a)
+    if a.config4 != "off":
+        out["config4"] = config4(a)
     if a.config5 != "off":
         out["config5"] = config5(a, int(a.config5), gnn)
     print(json.dumps(out))
@@ -223,6 +231,75 @@ def main():
 def sa_flux_algorithmic_bytes(nc, nbf, nf):
     # the viscous count plus the eddy viscosity of cells and ghosts (8 B each)
     return flux_algorithmic_bytes(nc, nbf, nf, viscous=True) + 8 * (nc + nbf)
+
+
+def equal_drop(s, U0, forms, N, out, tag, reset=None):
+    """The first form takes N implicit steps from U0 and fixes the relative
+    drop it reaches; every other form is timed from U0 to that drop (at most
+    4 N steps).  Adds a row per form and the speedups over the first to out."""
+    import cfdg.cases as cases
+
+    lp = cases.default_linparams()
+    target = None
+    for name, prec, net in forms:
+        s.set_precision(prec)
+        s.set_state(U0)
+        if reset is not None:
+            reset()
+        s.set_warm_start(net)
+        if target is None:
+            r = s.run_steady(lp, N, rel_tol=0.0, check_interval=1)
+            target = r.rel
+        else:
+            r = s.run_steady(lp, 4 * N, rel_tol=target, check_interval=1)
+        row = {"seconds": round(r.seconds, 3), "steps": int(r.steps), "linear_iterations": int(r.linear_iterations),
+               "rel": r.rel, "reached": bool(r.rel <= target), "ms_per_step": round(1e3 * r.seconds / max(r.steps, 1), 2)}
+        if net is not None:
+            row["warm_start_kept_rejected"] = list(s.warm_start_stats())
+        out[name] = row
+        print(f"bench.py: {tag} {name}: {row}", file=sys.stderr, flush=True)
+    s.set_warm_start(None)
+    out["drop"] = target
+    first = out[forms[0][0]]["seconds"]
+    for name, _, _ in forms[1:]:
+        out[f"speedup_{name}"] = round(first / out[name]["seconds"], 3)
+
+
+def config4(a):
+    """The learned warm start on a 4.19M-cell plate: 256 x 64 x 256
+    hexahedra, the laminar plate's box, stretching (1.08), 10 % jitter, RCM
+    order and flow (Mach 0.3, Re_L 1e5), second-order unlimited MUSCL with the
+    node LSQ gradient, CFL 100, from the freestream with 2 % seeded noise.
+    The network is trained first on 8 seeded 32 x 16 x 8 plates under the
+    same numerics (train_network).  The plain FP64 run takes a fixed number
+    of implicit steps; FP32 Krylov with a zero initial guess and with the
+    learned guess (zero guess as fallback, unchanged linear tolerance) are
+    timed to the drop it reached, from the same state."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    ni, nj, nk = (int(x) for x in a.config4.split("x"))
+    gnn, training = train_network(a.config4_cfl)
+    print(f"bench.py: config4 training {training}", file=sys.stderr, flush=True)
+    case = cases.plate(ni=ni, nj=nj, nk=nk, limiter="UNLIMITED", cfl=a.config4_cfl)
+    mesh = cfdg.Mesh.generate(**case.mesh)
+    s = cfdg.Solver(mesh, case.numerics, case.bcs)
+    s.init_uniform(case.rho, case.velocity, case.p)
+    U = s.get_state()
+    U0 = U * (1.0 + 0.02 * np.random.default_rng(3).standard_normal(U.shape))
+    print(f"bench.py: config4 mesh and context {mesh.n_cells} cells", file=sys.stderr, flush=True)
+    fp32 = cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1)
+    out = {"workload": f"laminar plate {ni}x{nj}x{nk} hexahedra ({mesh.n_cells} cells), Mach 0.3, Re_L 1e5, "
+                       f"unlimited MUSCL + node LSQ, CFL {a.config4_cfl:g}, BiCGSTAB + multicolour SGS (rel 0.1), "
+                       "2% seeded noise on the freestream", "training": training}
+    forms = [("plain_fp64", cfdg.Precision.make(), None), ("fp32_krylov", fp32, None),
+             ("fp32_krylov_learned", fp32, gnn)]
+    equal_drop(s, U0, forms, a.config4_steps, out, "config4")
+    out["learned_over_zero_guess"] = round(out["fp32_krylov"]["seconds"] / out["fp32_krylov_learned"]["seconds"], 3)
+    s.close()
+    return out
 
 
 def config5(a, n, gnn):
@@ -283,7 +360,6 @@ def config5(a, n, gnn):
                              "achieved": round(B / (flux_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                              "frac": round(B / (flux_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     print(f"bench.py: config5 residual {out['residual']}", file=sys.stderr, flush=True)
-    lp = cases.default_linparams()
     forms = [("plain_fp64", cfdg.Precision.make(), None),
              ("fp32_krylov", cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8, inner_rel_tol=0.1), None),
              ("bf16_blocks_fp32_krylov", cfdg.Precision.make(cfdg.PREC_BF16, cfdg.PREC_FP32, max_outer=8,
@@ -291,29 +367,7 @@ def config5(a, n, gnn):
     if gnn is not None:
         forms.append(("fp32_krylov_learned", cfdg.Precision.make(cfdg.PREC_FP32, cfdg.PREC_FP32, max_outer=8,
                                                                  inner_rel_tol=0.1), gnn))
-    target = None
-    N = a.config5_steps
-    for name, prec, net in forms:
-        s.set_precision(prec)
-        s.set_state(U0)
-        s.set_turbulence_state(rnu0)
-        s.set_warm_start(net)
-        if target is None:
-            r = s.run_steady(lp, N, rel_tol=0.0, check_interval=1)
-            target = r.rel
-        else:
-            r = s.run_steady(lp, 4 * N, rel_tol=target, check_interval=1)
-        row = {"seconds": round(r.seconds, 3), "steps": int(r.steps), "linear_iterations": int(r.linear_iterations),
-               "rel": r.rel, "reached": bool(r.rel <= target), "ms_per_step": round(1e3 * r.seconds / max(r.steps, 1), 2)}
-        if net is not None:
-            row["warm_start_kept_rejected"] = list(s.warm_start_stats())
-        out[name] = row
-        print(f"bench.py: config5 {name}: {row}", file=sys.stderr, flush=True)
-    s.set_warm_start(None)
-    out["drop"] = target
-    t_plain = out["plain_fp64"]["seconds"]
-    for name, _, _ in forms[1:]:
-        out[f"speedup_{name}"] = round(t_plain / out[name]["seconds"], 3)
+    equal_drop(s, U0, forms, a.config5_steps, out, "config5", lambda: s.set_turbulence_state(rnu0))
     s.close()
     return out
 
@@ -397,6 +451,38 @@ def drop_profile(a):
     return out
 
 
+def train_network(cfl=1000.0):
+    """The learned warm start's network, trained on the GPU: 8 seeded
+    32 x 16 x 8 plates with Mach drawn from U(0.2, 0.5), unlimited MUSCL at
+    the given CFL, 2 % noise on the freestream, every 100th of 2000 implicit
+    steps recorded, 30 epochs.  Returns the network and a summary."""
+    import numpy as np
+
+    import cfdg
+    import cfdg.cases as cases
+
+    lp = cases.default_linparams()
+    t0 = time.perf_counter()
+    gnn = cfdg.Gnn(seed=11)
+    rng = np.random.default_rng(2024)
+    for m in range(8):
+        tcase = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m,
+                            limiter="UNLIMITED", cfl=cfl)
+        mesh = cfdg.Mesh.generate(**tcase.mesh)
+        s = cfdg.Solver(mesh, tcase.numerics, tcase.bcs)
+        s.init_uniform(tcase.rho, tcase.velocity, tcase.p)
+        U = s.get_state()
+        s.set_state(U * (1.0 + 0.02 * np.random.default_rng(m).standard_normal(U.shape)))
+        s.set_recording(True)
+        for it in range(2000):
+            s.implicit_step(lp)
+            if it % 100 == 99:
+                gnn.record(s)
+        s.close()
+    loss = gnn.train(epochs=30, lr=1e-3, seed=5)
+    return gnn, {"seconds": round(time.perf_counter() - t0, 2), "samples": gnn.num_samples(), "final_loss": loss}
+
+
 def time_to_drop(a):
     """Wall time to a 1e-8 drop of the reference's relative residual
     (max_v L2_v / L2_v of the first step, solver.hpp:740-765) on a steady
@@ -468,28 +554,17 @@ def time_to_drop(a):
     print(f"bench.py: drop plain_fp64 {out['plain_fp64']}", file=sys.stderr, flush=True)
     conv = {}
     out["fp32_krylov"], conv["fp32_krylov"] = run("fp32")
+    print(f"bench.py: drop fp32_krylov {out['fp32_krylov']}", file=sys.stderr, flush=True)
     out["bf16_blocks_fp32_krylov"], conv["bf16_blocks_fp32_krylov"] = run("bf16")
-    print(f"bench.py: drop mixed forms done", file=sys.stderr, flush=True)
+    print(f"bench.py: drop bf16_blocks_fp32_krylov {out['bf16_blocks_fp32_krylov']}", file=sys.stderr, flush=True)
     if a.drop_previous_update:
         zero = cfdg.Gnn(seed=1)
         zero.set_params(np.zeros(zero.num_params, np.float32))
         out["fp32_krylov_previous_update"], _ = run("fp32", zero)
-    t0 = time.perf_counter()
-    gnn = cfdg.Gnn(seed=11)
-    rng = np.random.default_rng(2024)
-    for m in range(8):
-        tcase = cases.plate(ni=32, nj=16, nk=8, mach=float(rng.uniform(0.2, 0.5)), seed=100 + m, **kw)
-        s = start(tcase, m)
-        s.set_recording(True)
-        for it in range(2000):
-            s.implicit_step(lp)
-            if it % 100 == 99:
-                gnn.record(s)
-        s.close()
-    loss = gnn.train(epochs=30, lr=1e-3, seed=5)
-    out["training"] = {"seconds": round(time.perf_counter() - t0, 2), "samples": gnn.num_samples(),
-                       "final_loss": loss}
+    gnn, out["training"] = train_network()
+    print(f"bench.py: drop training {out['training']}", file=sys.stderr, flush=True)
     out["fp32_krylov_learned"], conv["fp32_krylov_learned"] = run("fp32", gnn)
+    print(f"bench.py: drop fp32_krylov_learned {out['fp32_krylov_learned']}", file=sys.stderr, flush=True)
     # converged fields against the plain FP64 run's: rho, p against the
     # freestream values, u, v, w against the freestream speed; wall force relative
     speed = float(np.linalg.norm(case.velocity))
