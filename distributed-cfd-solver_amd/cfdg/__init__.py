@@ -48,6 +48,16 @@ class MeshSpec(C.Structure):
     ]
 
 
+class ConnectivityView(C.Structure):
+    _fields_ = [
+        ("n_cells", _i32), ("has_node_stencil", _i32),
+        ("cell_faces_off", _p(_i32)), ("cell_faces", _p(_i32)),
+        ("cell_cells_face_off", _p(_i32)), ("cell_cells_face", _p(_i32)),
+        ("cell_cells_node_off", _p(_i32)), ("cell_cells_node", _p(_i32)),
+        ("bsr_row_ptr", _p(_i32)), ("bsr_cols", _p(_i32)), ("bsr_diag", _p(_i32)),
+    ]
+
+
 class MeshView(C.Structure):
     _fields_ = [
         ("n_cells", _i32), ("n_nodes", _i32), ("n_faces", _i32), ("n_inner_faces", _i32), ("n_patches", _i32),
@@ -153,6 +163,7 @@ def _load():
     sig = {
         "cfdg_mesh_generate": [vp, vp], "cfdg_mesh_from_view": [vp, vp], "cfdg_mesh_get_view": [vp, vp],
         "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp], "cfdg_mesh_geometry_device": [vp],
+        "cfdg_mesh_connectivity": [vp, _i32, _i32], "cfdg_mesh_get_connectivity": [vp, vp],
         "cfdg_mesh_cell_gid": [vp, vp],
         "cfdg_create": [vp, vp, vp, _i32, vp], "cfdg_destroy": [vp], "cfdg_sizes": [vp, vp, vp, vp],
         "cfdg_set_state": [vp, vp, _i64], "cfdg_get_state": [vp, vp, _i64], "cfdg_init_uniform": [vp, _d, vp, _d],
@@ -333,6 +344,37 @@ class Mesh:
         if rc != OK:
             raise CfdgError(rc, "reorder failed")
         lib.cfdg_mesh_get_view(self._h, C.byref(self.view))
+
+    def build_connectivity(self, on_device=True, node_stencil=True):
+        """Builds the cell connectivity and BSR pattern once for the mesh, on
+        the device or the host (cfdg_mesh_connectivity); solvers created from
+        the mesh afterwards use it."""
+        rc = lib.cfdg_mesh_connectivity(self._h, 1 if on_device else 0, 1 if node_stencil else 0)
+        if rc != OK:
+            raise CfdgError(rc, "connectivity failed")
+
+    def connectivity(self):
+        """Copies of the built connectivity arrays as numpy (int32)."""
+        v = ConnectivityView()
+        rc = lib.cfdg_mesh_get_connectivity(self._h, C.byref(v))
+        if rc != OK:
+            raise CfdgError(rc, "no connectivity built")
+        n = int(v.n_cells)
+
+        def get(ptr, k):
+            return np.ctypeslib.as_array(ptr, shape=(int(k),)).copy() if k > 0 else np.zeros(0, np.int32)
+
+        out = {}
+        pairs = [("cell_faces", v.cell_faces_off, v.cell_faces), ("cell_cells_face", v.cell_cells_face_off, v.cell_cells_face),
+                 ("bsr", v.bsr_row_ptr, v.bsr_cols)]
+        if v.has_node_stencil:
+            pairs.append(("cell_cells_node", v.cell_cells_node_off, v.cell_cells_node))
+        for name, po, pi in pairs:
+            off = get(po, n + 1)
+            out[name + "_off"] = off
+            out[name] = get(pi, off[-1])
+        out["bsr_diag"] = get(v.bsr_diag, n)
+        return out
 
     def compute_geometry_on_device(self):
         """Recomputes the cell and face metrics from the nodes on the device

@@ -639,7 +639,8 @@ void check_numerics(const cfdg_numerics* num) {
         throw CfdgError(CFDG_ERR_CONFIG, "viscous flow needs prandtl > 0 and prandtl_turb > 0");
 }
 
-void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, const cfdg_bc* bcs, int32_t n_bcs) {
+void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg::MeshConnectivity* cn, const cfdg_numerics* num,
+                   const cfdg_bc* bcs, int32_t n_bcs) {
     if (n_bcs != static_cast<int32_t>(m.patch_names.size()))
         throw CfdgError(CFDG_ERR_ARG, "one boundary condition per mesh patch is required");
     if (n_bcs > cfdg::kMaxPatches) throw CfdgError(CFDG_ERR_ARG, "too many patches");
@@ -669,9 +670,14 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     for (int32_t p = 0; p < n_bcs; ++p) c.bcs.bc[p] = resolve_bc(bcs[p], num->gamma, num->gas_constant);
     hipStream_t s = c.stream;
 
-    // mesh-numbered connectivity, device tiling and permutation
-    const cfdg::Csr cf_mesh = cfdg::cell_faces(m);
-    c.ccf_mesh = cfdg::cell_cells_by_face(m);
+    // mesh-numbered connectivity (built with the mesh when it holds one),
+    // device tiling and permutation
+    if (cn && (cn->cell_faces.off.size() != z(c.nc) + 1 || cn->bsr.diag.size() != z(c.nc)))
+        throw CfdgError(CFDG_ERR_STATE, "the mesh's connectivity does not match its cells");
+    cfdg::Csr cf_own;
+    if (!cn) cf_own = cfdg::cell_faces(m);
+    const cfdg::Csr& cf_mesh = cn ? cn->cell_faces : cf_own;
+    c.ccf_mesh = cn ? cn->cell_cells_face : cfdg::cell_cells_by_face(m);
     const cfdg::Tiling tl = cfdg::tile_cells(m, c.ccf_mesh, cf_mesh, cfdg::kTileCells, cfdg::kTileFaces);
     c.new2old = tl.new2old;
     c.old2new = tl.old2new;
@@ -753,7 +759,10 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
         c.sync();
     }
     if (num->gradient == CFDG_LEAST_SQUARES_NODE) {
-        const cfdg::Csr st = to_device_rows(cfdg::cell_cells_by_node(m), c.new2old, &c.old2new);
+        cfdg::Csr st_own;
+        if (!(cn && cn->has_node_stencil)) st_own = cfdg::cell_cells_by_node(m);
+        const cfdg::Csr st = to_device_rows(cn && cn->has_node_stencil ? cn->cell_cells_node : st_own, c.new2old,
+                                            &c.old2new);
         c.ccn_off.upload(st.off, s);
         c.ccn.upload(st.idx, s);
         // LDS-staged stencils: the flux tiles when their cells + halo fit the
@@ -830,7 +839,9 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg_numerics* num, c
     }
     // BSR: the mesh-order pattern (bsr_helpers.hpp:16-59) and its device rows
     {
-        const cfdg::BsrPattern bp = cfdg::bsr_pattern(c.ccf_mesh, c.nc);
+        cfdg::BsrPattern bp_own;
+        if (!cn) bp_own = cfdg::bsr_pattern(c.ccf_mesh, c.nc);
+        const cfdg::BsrPattern& bp = cn ? cn->bsr : bp_own;
         c.h_row_ptr = bp.row_ptr;
         c.h_cols = bp.cols;
         c.h_diag = bp.diag;
@@ -1024,7 +1035,7 @@ int cfdg_create(const cfdg_mesh* mesh, const cfdg_numerics* num, const cfdg_bc* 
     if (!mesh || !num || !out || (n_bcs > 0 && !bcs)) return CFDG_ERR_ARG;
     *out = nullptr;
     auto* c = new cfdg_ctx();
-    const int rc = guarded(c, [&] { setup_context(*c, mesh->m, num, bcs, n_bcs); });
+    const int rc = guarded(c, [&] { setup_context(*c, mesh->m, mesh->conn.get(), num, bcs, n_bcs); });
     if (rc != CFDG_OK) {
         std::fprintf(stderr, "cfdg_create: %s\n", c->err.c_str());
         delete c;
@@ -1457,8 +1468,10 @@ int cfdg_set_turbulence(cfdg_ctx* c, const cfdg_mesh* mesh, const cfdg_turbulenc
             kind[z(b)] = (p >= 0 && p < c->npatch) ? zone[z(p)] : cfdg::kZoneSymmetry;
         }
         // wall distance over the vertex stencil (solve_wall_distance, spalart_allmaras.hpp:183-189)
-        c->h_sa_dist = cfdg::wall_distance(m, cfdg::cell_cells_by_node(m), wall, t->max_distance_sweeps,
-                                           t->distance_tolerance);
+        const bool own_stencil = !(mesh->conn && mesh->conn->has_node_stencil);
+        const cfdg::Csr st = own_stencil ? cfdg::cell_cells_by_node(m) : cfdg::Csr{};
+        c->h_sa_dist = cfdg::wall_distance(m, own_stencil ? st : mesh->conn->cell_cells_node, wall,
+                                           t->max_distance_sweeps, t->distance_tolerance);
         std::vector<double> dist_dev(z(c->nc));
         for (int r = 0; r < c->nc; ++r) dist_dev[z(r)] = c->h_sa_dist[z(c->new2old[z(r)])];
         // freestream working values (spalart_allmaras.hpp:191-193)

@@ -4,10 +4,16 @@
 #include <stdexcept>
 #include <string>
 
+#include <memory>
+
 #include "mesh.hpp"
+#include "setup.hpp"
 
 struct cfdg_mesh {
     cfdg::Mesh m;
+    // cell connectivity built by cfdg_mesh_connectivity (dropped when the
+    // cells are renumbered); contexts created from the mesh use it
+    std::shared_ptr<const cfdg::MeshConnectivity> conn;
 };
 
 namespace cfdg {

@@ -1,6 +1,7 @@
 // C ABI of the host-side mesh objects (include/cfdg.h, cfdg_mesh_*).
 #include <algorithm>
 #include <cstdio>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -281,6 +282,7 @@ int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh) {
     if (!mesh) return CFDG_ERR_ARG;
     try {
         cfdg::apply_cell_permutation(mesh->m, cfdg::rcm_order(mesh->m));
+        mesh->conn.reset();
         return CFDG_OK;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "cfdg_mesh_reorder_rcm: %s\n", e.what());
@@ -303,6 +305,40 @@ int cfdg_mesh_geometry_device(cfdg_mesh* mesh) {
         std::fprintf(stderr, "cfdg_mesh_geometry_device: %s\n", e.what());
         return CFDG_ERR_MESH;
     }
+}
+
+int cfdg_mesh_connectivity(cfdg_mesh* mesh, int32_t on_device, int32_t node_stencil) {
+    if (!mesh) return CFDG_ERR_ARG;
+    try {
+        const bool ns = node_stencil != 0;
+        mesh->conn = std::make_shared<const cfdg::MeshConnectivity>(
+            on_device ? cfdg::device_connectivity(mesh->m, ns) : cfdg::host_connectivity(mesh->m, ns));
+        return CFDG_OK;
+    } catch (const cfdg::DeviceError& e) {
+        std::fprintf(stderr, "cfdg_mesh_connectivity: %s\n", e.what());
+        return CFDG_ERR_HIP;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "cfdg_mesh_connectivity: %s\n", e.what());
+        return CFDG_ERR_MESH;
+    }
+}
+
+int cfdg_mesh_get_connectivity(const cfdg_mesh* mesh, cfdg_mesh_connectivity_view* v) {
+    if (!mesh || !v) return CFDG_ERR_ARG;
+    if (!mesh->conn) return CFDG_ERR_STATE;
+    const cfdg::MeshConnectivity& c = *mesh->conn;
+    v->n_cells = mesh->m.n_cells;
+    v->has_node_stencil = c.has_node_stencil ? 1 : 0;
+    v->cell_faces_off = c.cell_faces.off.data();
+    v->cell_faces = c.cell_faces.idx.data();
+    v->cell_cells_face_off = c.cell_cells_face.off.data();
+    v->cell_cells_face = c.cell_cells_face.idx.data();
+    v->cell_cells_node_off = c.has_node_stencil ? c.cell_cells_node.off.data() : nullptr;
+    v->cell_cells_node = c.has_node_stencil ? c.cell_cells_node.idx.data() : nullptr;
+    v->bsr_row_ptr = c.bsr.row_ptr.data();
+    v->bsr_cols = c.bsr.cols.data();
+    v->bsr_diag = c.bsr.diag.data();
+    return CFDG_OK;
 }
 
 void cfdg_mesh_free(cfdg_mesh* mesh) { delete mesh; }

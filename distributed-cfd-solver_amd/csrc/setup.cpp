@@ -201,6 +201,16 @@ BsrPattern bsr_pattern(const Csr& ccf, i32 n) {
     return p;
 }
 
+MeshConnectivity host_connectivity(const Mesh& m, bool node_stencil) {
+    MeshConnectivity c;
+    c.cell_faces = cell_faces(m);
+    c.cell_cells_face = cell_cells_by_face(m);
+    c.bsr = bsr_pattern(c.cell_cells_face, m.n_cells);
+    if (node_stencil) c.cell_cells_node = cell_cells_by_node(m);
+    c.has_node_stencil = node_stencil;
+    return c;
+}
+
 std::vector<i32> greedy_colours(const Csr& ccf, i32 n, i32& n_colours) {
     std::vector<i32> col(z(n), -1);
     n_colours = 0;

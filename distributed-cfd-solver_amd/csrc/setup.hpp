@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 
 #include "mesh.hpp"
@@ -41,6 +42,27 @@ struct BsrPattern {
 };
 // Row i = sort({i} U face neighbours of i) (bsr_helpers.hpp:16-59).
 BsrPattern bsr_pattern(const Csr& ccf, std::int32_t n);
+
+// The cell connectivity a context is built from: the faces of each cell,
+// its face and vertex neighbours and the BSR pattern of the Jacobian
+// (MeshAuxConnectivity, aux_connectivity.cpp:11-187, 348-445, and
+// bsr_helpers.hpp:16-59).  The vertex stencil is built only on request
+// (node-LSQ gradients and the wall distance use it).
+struct MeshConnectivity {
+    Csr cell_faces, cell_cells_face, cell_cells_node;
+    BsrPattern bsr;
+    bool has_node_stencil = false;
+};
+// The builders above, on the host.
+MeshConnectivity host_connectivity(const Mesh& m, bool node_stencil);
+// A HIP runtime failure of a device-side set-up step.
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+// The same arrays built on the current HIP device (kernels_conn.hip), equal
+// to host_connectivity's entry for entry; throws DeviceError on a HIP failure
+// and std::runtime_error on a mesh whose faces couple a pair of cells twice.
+MeshConnectivity device_connectivity(const Mesh& m, bool node_stencil);
 
 // Row groups for the parallel Gauss-Seidel sweeps: rows of one group have no
 // matrix coupling, so they can be updated together.

@@ -174,6 +174,35 @@ int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh);
    nodes.  CFDG_ERR_MESH for a degenerate cell or face (volume or area not
    above 1e-15), the mesh then left unchanged. */
 int cfdg_mesh_geometry_device(cfdg_mesh* mesh);
+/* Builds the cell connectivity the reference's MeshAuxConnectivity holds
+   (src/mesh/aux_connectivity.cpp:11-187, 348-445) and the Jacobian's BSR
+   pattern (bsr_helpers.hpp:16-59) once for the mesh: the faces of each cell
+   in ascending face index, the face neighbours and (node_stencil != 0) the
+   vertex neighbours of each cell in ascending order, and the BSR rows
+   sort({i} U face neighbours).  on_device != 0 builds them on the current
+   HIP device (sorted 64-bit keys), otherwise on the host; both give the same
+   arrays entry for entry.  Contexts created from the mesh afterwards use
+   them instead of building their own; cfdg_mesh_reorder_rcm drops them.
+   CFDG_ERR_MESH when two faces couple the same pair of cells (as context
+   creation reports it), CFDG_ERR_HIP on a device failure. */
+int cfdg_mesh_connectivity(cfdg_mesh* mesh, int32_t on_device, int32_t node_stencil);
+typedef struct {
+    int32_t n_cells;
+    int32_t has_node_stencil;
+    const int32_t* cell_faces_off;      /* n_cells + 1 */
+    const int32_t* cell_faces;          /* cell_faces_off[n_cells] */
+    const int32_t* cell_cells_face_off; /* n_cells + 1 */
+    const int32_t* cell_cells_face;
+    const int32_t* cell_cells_node_off; /* n_cells + 1, NULL without the vertex stencil */
+    const int32_t* cell_cells_node;
+    const int32_t* bsr_row_ptr;         /* n_cells + 1 */
+    const int32_t* bsr_cols;            /* bsr_row_ptr[n_cells] */
+    const int32_t* bsr_diag;            /* n_cells */
+} cfdg_mesh_connectivity_view;
+/* Borrowed view of the connectivity built by cfdg_mesh_connectivity (valid
+   until the mesh is freed, renumbered or given a new connectivity);
+   CFDG_ERR_STATE when none is built. */
+int cfdg_mesh_get_connectivity(const cfdg_mesh* mesh, cfdg_mesh_connectivity_view* view);
 void cfdg_mesh_free(cfdg_mesh* mesh);
 
 /* ---- solver context ----------------------------------------------------- */
