@@ -352,6 +352,16 @@ __global__ void __launch_bounds__(kBlock) k_gradient_limiter(MeshArgs m, FieldAr
         });
 }
 
+// Four consecutive 16-bit LDS slot indices of a stencil list, loaded as one
+// 8-byte word; entry k of the chunk sits in bits 16 (k mod 4) and up.  Taking
+// an entry by shifting the word (not by picking one of two 32-bit halves)
+// keeps the chunk in registers: a pick between the halves by a run-time index
+// was compiled into an indexed load from a stack copy.
+typedef unsigned long long SlotChunk;
+__device__ __forceinline__ int chunk_slot(SlotChunk ch, int k) {
+    return static_cast<int>((ch >> ((k & 3) * 16)) & 0xffffULL);
+}
+
 // View of one LDS slot of the staged gradient kernel: planes 0-7 (q,
 // centroid) stored as interleaved pairs like the cell planes, quantity k of
 // slot s at sq[((k / 2) kSlots + s) 2 + k % 2], so one 16-byte LDS read
@@ -381,9 +391,9 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
     const bool live = static_cast<int>(threadIdx.x) < nt;
     const int nchn = (m.ccn_base[t + 1] - m.ccn_base[t]) / (4 * nt);
     const int nchf = (m.ccf_base[t + 1] - m.ccf_base[t]) / (4 * nt);
-    const uint2* chn = reinterpret_cast<const uint2*>(m.ccn_slot + m.ccn_base[t]) + threadIdx.x;
-    const uint2* chf = reinterpret_cast<const uint2*>(m.ccf_slot + m.ccf_base[t]) + threadIdx.x;
-    uint2 curn = make_uint2(0u, 0u), nxtn = curn, curf = curn, nxtf = curn;
+    const SlotChunk* chn = reinterpret_cast<const SlotChunk*>(m.ccn_slot + m.ccn_base[t]) + threadIdx.x;
+    const SlotChunk* chf = reinterpret_cast<const SlotChunk*>(m.ccf_slot + m.ccf_base[t]) + threadIdx.x;
+    SlotChunk curn = 0, nxtn = 0, curf = 0, nxtf = 0;
     if (live) {
         if (nchn > 0) curn = chn[0];
         if (nchn > 1) nxtn = chn[nt];
@@ -402,13 +412,12 @@ __global__ void __launch_bounds__(kTileCells) k_gradient_limiter_lds(MeshArgs m,
     if (!live) return;
     // entries are taken in order k = 0, 1, 2, ...: at each chunk boundary the
     // held next chunk becomes current and the one after it is requested
-    const auto take = [&](int k, uint2& cur, uint2& nxt, const uint2* ch, int nch) {
+    const auto take = [&](int k, SlotChunk& cur, SlotChunk& nxt, const SlotChunk* ch, int nch) {
         if ((k & 3) == 0 && k > 0) {
             cur = nxt;
             if ((k >> 2) + 1 < nch) nxt = ch[((k >> 2) + 1) * nt];
         }
-        const unsigned w = (k & 2) ? cur.y : cur.x;
-        return static_cast<int>((w >> ((k & 1) * 16)) & 0xffffu);
+        return chunk_slot(cur, k);
     };
     const auto nbn = [&](int, int k) { return LdsSlot<kSlots>{sq, take(k, curn, nxtn, chn, nchn)}; };
     const auto nbf = [&](int, int k) { return LdsSlot<kSlots>{sq, take(k, curf, nxtf, chf, nchf)}; };
@@ -453,9 +462,9 @@ __global__ void __launch_bounds__(2 * kTileCells) k_gradient_limiter_pair(MeshAr
     const int cell = static_cast<int>(threadIdx.x >> 1), half = static_cast<int>(threadIdx.x & 1);
     const bool live = cell < nt;
     const int nchn = (m.ccn_base[t + 1] - m.ccn_base[t]) / (4 * nt);
-    const uint2* chn = reinterpret_cast<const uint2*>(m.ccn_slot + m.ccn_base[t]) + cell;
+    const SlotChunk* chn = reinterpret_cast<const SlotChunk*>(m.ccn_slot + m.ccn_base[t]) + cell;
     // this lane's first two vertex-slot chunks, requested before the staging
-    uint2 cur = make_uint2(0u, 0u), nxt = cur;
+    SlotChunk cur = 0, nxt = 0;
     if (live) {
         if (half < nchn) cur = chn[half * nt];
         if (half + 2 < nchn) nxt = chn[(half + 2) * nt];
@@ -483,12 +492,11 @@ __global__ void __launch_bounds__(2 * kTileCells) k_gradient_limiter_pair(MeshAr
         const double i0 = ri[kRInv], i1 = ri[kRInv + 1], i2 = ri[kRInv + 2], i4 = ri[kRInv + 3], i5 = ri[kRInv + 4],
                      i8 = ri[kRInv + 5];
         for (int q = half; 4 * q < cnt; q += 2) {
-            const uint2 ch = cur;
+            const SlotChunk ch = cur;
             cur = nxt;
             if (q + 4 < nchn) nxt = chn[(q + 4) * nt];
             for (int e = 0; e < 4 && 4 * q + e < cnt; ++e) {
-                const unsigned w = (e & 2) ? ch.y : ch.x;
-                const int slot = static_cast<int>((w >> ((e & 1) * 16)) & 0xffffu);
+                const int slot = chunk_slot(ch, e);
                 const LdsSlot<kSlots> rn{sq, slot};
                 const double dx = rn[kRC] - x0, dy = rn[kRC + 1] - y0, dz = rn[kRC + 2] - z0;
                 const double r2 = dx * dx + dy * dy + dz * dz;
@@ -526,8 +534,8 @@ __global__ void __launch_bounds__(2 * kTileCells) k_gradient_limiter_pair(MeshAr
     }
     // the limiter's face neighbours and face vectors, in order, as the one-lane form reads them
     const int nchf = (m.ccf_base[t + 1] - m.ccf_base[t]) / (4 * nt);
-    const uint2* chf = reinterpret_cast<const uint2*>(m.ccf_slot + m.ccf_base[t]) + cell;
-    uint2 curf = make_uint2(0u, 0u), nxtf = curf;
+    const SlotChunk* chf = reinterpret_cast<const SlotChunk*>(m.ccf_slot + m.ccf_base[t]) + cell;
+    SlotChunk curf = 0, nxtf = 0;
     if (nchf > 0) curf = chf[0];
     if (nchf > 1) nxtf = chf[nt];
     const auto nbf = [&](int, int k) {
@@ -535,8 +543,7 @@ __global__ void __launch_bounds__(2 * kTileCells) k_gradient_limiter_pair(MeshAr
             curf = nxtf;
             if ((k >> 2) + 1 < nchf) nxtf = chf[((k >> 2) + 1) * nt];
         }
-        const unsigned w = (k & 2) ? curf.y : curf.x;
-        return LdsSlot<kSlots>{sq, static_cast<int>((w >> ((k & 1) * 16)) & 0xffffu)};
+        return LdsSlot<kSlots>{sq, chunk_slot(curf, k)};
     };
     const long long vb = m.face_vec_base[t];
     const long long vw = (m.face_vec_base[t + 1] - vb) / (3LL * nt);
