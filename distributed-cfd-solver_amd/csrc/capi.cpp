@@ -661,7 +661,7 @@ void setup_context(cfdg_ctx& c, const cfdg::Mesh& m, const cfdg::MeshConnectivit
     c.nbf = m.n_faces - m.n_inner_faces;
     c.ntot = c.nc + c.nbf;
     c.npatch = n_bcs;
-    c.nargs.gas = cfdg::Gas{num->gamma, num->gas_constant, num->prandtl, num->prandtl_turb};
+    c.nargs.gas = cfdg::make_gas(num->gamma, num->gas_constant, num->prandtl, num->prandtl_turb);
     c.nargs.viscous = num->flow_model == CFDG_VISCOUS_FLOW;
     c.nargs.muscl = num->reconstruction == CFDG_MUSCL;
     c.nargs.limiter = num->limiter;

@@ -21,10 +21,18 @@ struct Gas {
     double R;
     double prandtl;
     double prandtl_turb;
+    double cv, cp;  // R / (gamma - 1) and gamma cv, formed once by make_gas
 };
 
-CFDG_HD double gas_cv(const Gas& g) { return g.R / (g.gamma - 1.0); }
-CFDG_HD double gas_cp(const Gas& g) { return g.gamma * gas_cv(g); }
+// The gas with its specific heats formed once (the same IEEE operations on
+// the host as on the device, so kernels read them from their arguments
+// instead of forming them per thread).
+CFDG_HD Gas make_gas(double gamma, double R, double prandtl, double prandtl_turb) {
+    const double cv = R / (gamma - 1.0);
+    return Gas{gamma, R, prandtl, prandtl_turb, cv, gamma * cv};
+}
+CFDG_HD double gas_cv(const Gas& g) { return g.cv; }
+CFDG_HD double gas_cp(const Gas& g) { return g.cp; }
 
 CFDG_HD double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max semantics
 CFDG_HD double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min semantics
@@ -218,6 +226,10 @@ CFDG_HD double sutherland(double T) {
 // contribution.  qa/qb: (p,u,v,w,T) of the owner and neighbour (or ghost);
 // ga/gb: gradients of u, v, w, T, 3 components each ([du, dv, dw, dT]).
 // (dvx, dvy, dvz) is the owner->neighbour centroid vector, inv_d its inverse length.
+// kEddy = false is the form without eddy viscosity (mutL = mutR = 0): the
+// terms 0.5 (0 + 0) and (0 cp) / Pr_t it leaves out are +0, and adding +0 to
+// the positive laminar values changes no bit.
+template <bool kEddy = true>
 CFDG_HD void viscous_flux(const Gas& g, double inv_d, double dvx, double dvy, double dvz, const double qa[5],
                           const double qb[5], const double ga[12], const double gb[12], double mutL, double mutR,
                           double nx, double ny, double nz, double area, double Fv[5], double& lam_visc) {
@@ -227,11 +239,14 @@ CFDG_HD void viscous_flux(const Gas& g, double inv_d, double dvx, double dvy, do
     const double w_f = 0.5 * (qa[3] + qb[3]);
     const double T_f = 0.5 * (qa[4] + qb[4]);
     const double mu_lam = sutherland(T_f);
-    const double mu_t_f = 0.5 * (mutL + mutR);
-    const double mu_eff = mu_lam + mu_t_f;
     const double k_lam = (sutherland(T_f) * gas_cp(g)) / g.prandtl;
-    const double cp = gas_cp(g);
-    const double k_eff = k_lam + (mu_t_f * cp) / g.prandtl_turb;
+    double mu_eff = mu_lam, k_eff = k_lam;
+    if (kEddy) {
+        const double mu_t_f = 0.5 * (mutL + mutR);
+        mu_eff = mu_lam + mu_t_f;
+        const double cp = gas_cp(g);
+        k_eff = k_lam + (mu_t_f * cp) / g.prandtl_turb;
+    }
     const double xi_x = dvx * inv_d, xi_y = dvy * inv_d, xi_z = dvz * inv_d;
     const double ncx = nx - xi_x, ncy = ny - xi_y, ncz = nz - xi_z;
     const double du_dx = 0.5 * (ga[0] + gb[0]), du_dy = 0.5 * (ga[1] + gb[1]), du_dz = 0.5 * (ga[2] + gb[2]);

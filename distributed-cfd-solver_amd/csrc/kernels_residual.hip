@@ -649,11 +649,11 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
 // MUSCL states of muscl.hpp:108-157).
 // fr views the face's planes, (c0, c1) = (owner, neighbour or ghost).
 // r0 / r1 view the owner's and the other cell's planes (in HBM or staged in
-// LDS), mutL / mutR are their eddy viscosities (zero without turbulence).
+// LDS), mutL / mutR are their eddy viscosities (kEddy = false: none, both 0).
 // The owner -> neighbour centroid vector and its inverse length are formed
 // here with the operations of face_distances (aux_geometry.cpp:11-51,120-156),
 // so they equal the mesh-order face planes bit for bit.
-template <class FR, class R0, class R1>
+template <bool kEddy, class FR, class R0, class R1>
 __device__ __forceinline__ void face_flux(const NumArgs& nm, const FR& fr, bool inner, const R0& r0, const R1& r1,
                                           double mutL, double mutR, double F[5], double& lam_f) {
     const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
@@ -706,7 +706,7 @@ __device__ __forceinline__ void face_flux(const NumArgs& nm, const FR& fr, bool 
         }
         double Fv[5];
         double lv = 0.0;
-        viscous_flux(nm.gas, inv_d, dvx, dvy, dvz, qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A, Fv, lv);
+        viscous_flux<kEddy>(nm.gas, inv_d, dvx, dvy, dvz, qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A, Fv, lv);
         for (int v = 1; v < 5; ++v) F[v] += Fv[v];
         lam_f += lv;
     }
@@ -808,13 +808,13 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         const bool in0 = s0 < static_cast<unsigned>(nt), in1 = s1 < static_cast<unsigned>(nt);
         double* Fr = F[r];
         if (in0 && in1) {
-            face_flux(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, LdsRec{sc, static_cast<int>(s1)},
+            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, LdsRec{sc, static_cast<int>(s1)},
                       kTurb ? smut[s0] : 0.0, kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
         } else if (in0) {
-            face_flux(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, crec(f, cy[r]), kTurb ? smut[s0] : 0.0,
+            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, crec(f, cy[r]), kTurb ? smut[s0] : 0.0,
                       kTurb ? f.mut[cy[r]] : 0.0, Fr, Fr[5]);
         } else {
-            face_flux(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
+            face_flux<kTurb>(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
                       kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
         }
         if (kTurb) f.mdot_tile[fb + j] = Fr[0];
