@@ -277,6 +277,8 @@ CFDG_HD void viscous_flux(const Gas& g, double inv_d, double dvx, double dvy, do
 // primitive state; only the non-zero entries are produced:
 //   m[0..2]: row 1..3 first column (-C_mu u/rho, ...), m[3]: C_mu/rho (diagonal of rows 1..3),
 //   m[4..8]: row 4 (energy).
+// kEddy = false: no eddy viscosity, as in viscous_flux.
+template <bool kEddy = true>
 CFDG_HD void viscous_jacobian_entries(const Gas& g, double inv_d, const double qL[5], const double qR[5],
                                       double mutL, double mutR, double area, double m[9]) {
     const double p_f = 0.5 * (qL[0] + qR[0]);
@@ -289,11 +291,14 @@ CFDG_HD void viscous_jacobian_entries(const Gas& g, double inv_d, const double q
     const double cv = gas_cv(g);
     const double inv_rho_cv = 1.0 / (rho_f * cv);
     const double mu_lam = sutherland(T_f);
-    const double mu_t_f = 0.5 * (mutL + mutR);
-    const double mu_eff = mu_lam + mu_t_f;
     const double k_lam = (sutherland(T_f) * gas_cp(g)) / g.prandtl;
-    const double cp = gas_cp(g);
-    const double k_eff = k_lam + (mu_t_f * cp) / g.prandtl_turb;
+    double mu_eff = mu_lam, k_eff = k_lam;
+    if (kEddy) {
+        const double mu_t_f = 0.5 * (mutL + mutR);
+        mu_eff = mu_lam + mu_t_f;
+        const double cp = gas_cp(g);
+        k_eff = k_lam + (mu_t_f * cp) / g.prandtl_turb;
+    }
     const double C_mu = mu_eff * inv_d * area;
     const double C_lam = k_eff * inv_d * area;
     const double V2 = u_f * u_f + v_f * v_f + w_f * w_f;

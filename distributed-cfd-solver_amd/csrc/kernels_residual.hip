@@ -653,9 +653,9 @@ __global__ void __launch_bounds__(kBlock) k_ghost_gradients(MeshArgs m, FieldArg
 // The owner -> neighbour centroid vector and its inverse length are formed
 // here with the operations of face_distances (aux_geometry.cpp:11-51,120-156),
 // so they equal the mesh-order face planes bit for bit.
-template <bool kEddy, class FR, class R0, class R1>
+template <bool kEddy, class FR, class R0, class R1, class ML, class MR>
 __device__ __forceinline__ void face_flux(const NumArgs& nm, const FR& fr, bool inner, const R0& r0, const R1& r1,
-                                          double mutL, double mutR, double F[5], double& lam_f) {
+                                          const ML& mut0, const MR& mut1, double F[5], double& lam_f) {
     const double nx = fr[kFN], ny = fr[kFN + 1], nz = fr[kFN + 2], A = fr[kFA];
     double qL[5], qR[5];
     if (inner && nm.muscl) {
@@ -706,6 +706,7 @@ __device__ __forceinline__ void face_flux(const NumArgs& nm, const FR& fr, bool 
         }
         double Fv[5];
         double lv = 0.0;
+        const double mutL = kEddy ? mut0() : 0.0, mutR = kEddy ? mut1() : 0.0;  // read where they are used
         viscous_flux<kEddy>(nm.gas, inv_d, dvx, dvy, dvz, qa, qb, ga, gb, mutL, mutR, nx, ny, nz, A, Fv, lv);
         for (int v = 1; v < 5; ++v) F[v] += Fv[v];
         lam_f += lv;
@@ -807,15 +808,19 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         const unsigned s0 = static_cast<unsigned>(cx[r] - c0), s1 = static_cast<unsigned>(cy[r] - c0);
         const bool in0 = s0 < static_cast<unsigned>(nt), in1 = s1 < static_cast<unsigned>(nt);
         double* Fr = F[r];
+        // eddy viscosities from LDS (tile cells) or HBM (halo and ghosts)
+        const int g0 = cx[r], g1 = cy[r];
+        const auto lds0 = [&] { return smut[s0]; };
+        const auto lds1 = [&] { return smut[s1]; };
+        const auto hbm0 = [&] { return f.mut[g0]; };
+        const auto hbm1 = [&] { return f.mut[g1]; };
         if (in0 && in1) {
-            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, LdsRec{sc, static_cast<int>(s1)},
-                      kTurb ? smut[s0] : 0.0, kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
+            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, LdsRec{sc, static_cast<int>(s1)}, lds0,
+                             lds1, Fr, Fr[5]);
         } else if (in0) {
-            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, crec(f, cy[r]), kTurb ? smut[s0] : 0.0,
-                      kTurb ? f.mut[cy[r]] : 0.0, Fr, Fr[5]);
+            face_flux<kTurb>(nm, fr, inner, LdsRec{sc, static_cast<int>(s0)}, crec(f, g1), lds0, hbm1, Fr, Fr[5]);
         } else {
-            face_flux<kTurb>(nm, fr, inner, crec(f, cx[r]), LdsRec{sc, static_cast<int>(s1)}, kTurb ? f.mut[cx[r]] : 0.0,
-                      kTurb ? smut[s1] : 0.0, Fr, Fr[5]);
+            face_flux<kTurb>(nm, fr, inner, crec(f, g0), LdsRec{sc, static_cast<int>(s1)}, hbm0, lds1, Fr, Fr[5]);
         }
         if (kTurb) f.mdot_tile[fb + j] = Fr[0];
         if (!inner)  // a boundary face lies in exactly one tile: single writer; ghost id = cells + boundary index
@@ -1085,7 +1090,7 @@ __device__ __forceinline__ bool jacobian_face(const MeshArgs& m, const FieldArgs
         const double mc = kTurb ? smut[i] : 0.0;
         const double mutL = kTurb ? (own ? mc : mo) : 0.0;
         const double mutR = (kTurb && inner) ? (own ? mo : mc) : 0.0;
-        viscous_jacobian_entries(nm.gas, inv_d, qL, qR, mutL, mutR, A, vm);
+        viscous_jacobian_entries<kTurb>(nm.gas, inv_d, qL, qR, mutL, mutR, A, vm);
     }
     // own side, dFL = 0.5 A J(UL) + diss I (+ viscous) as owner, dFR = 0.5 A
     // J(UR) - diss I (- viscous) as neighbour
@@ -1122,7 +1127,8 @@ __device__ __forceinline__ bool jacobian_face(const MeshArgs& m, const FieldArgs
 // The arithmetic and its order are those of the reference's face loop, so the
 // blocks are the reference's bit for bit; each block has a single writer.
 template <class TS, bool kTurb>
-__global__ void __launch_bounds__(kTileCells) k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
+__global__ void __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? 2 : 3)))
+k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
                                                          const double* alpha, TS* values) {
     typedef typename BlockWords<TS>::word word;
     constexpr int kW = BlockWords<TS>::n;
