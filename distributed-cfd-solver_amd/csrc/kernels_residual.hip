@@ -1126,8 +1126,13 @@ __device__ __forceinline__ bool jacobian_face(const MeshArgs& m, const FieldArgs
 // contiguous runs instead of 64 scattered words.
 // The arithmetic and its order are those of the reference's face loop, so the
 // blocks are the reference's bit for bit; each block has a single writer.
+// Waves per SIMD: three where the LDS leaves room for three workgroups per
+// CU (FP32 blocks without eddy viscosity, BF16 blocks: 52 KB and 45 KB), at
+// 168 registers; two otherwise (FP64 blocks: 78 KB; FP32 with eddy
+// viscosity: 54 KB), where a lower register cap would only add spills.
 template <class TS, bool kTurb>
-__global__ void __launch_bounds__(kTileCells) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? 2 : 3)))
+__global__ void __launch_bounds__(kTileCells)
+__attribute__((amdgpu_waves_per_eu((sizeof(TS) == 8 || (sizeof(TS) == 4 && kTurb)) ? 2 : 3)))
 k_jacobian(MeshArgs m, FieldArgs f, NumArgs nm, BsrArgs b,
                                                          const double* alpha, TS* values) {
     typedef typename BlockWords<TS>::word word;
