@@ -162,7 +162,7 @@ def _load():
     vp = C.c_void_p
     sig = {
         "cfdg_mesh_generate": [vp, vp], "cfdg_mesh_from_view": [vp, vp], "cfdg_mesh_get_view": [vp, vp],
-        "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp], "cfdg_mesh_geometry_device": [vp],
+        "cfdg_mesh_patch_name": [vp, _i32, C.c_char_p, _i32], "cfdg_mesh_reorder_rcm": [vp], "cfdg_mesh_reorder_rcm_device": [vp], "cfdg_mesh_geometry_device": [vp],
         "cfdg_mesh_connectivity": [vp, _i32, _i32], "cfdg_mesh_get_connectivity": [vp, vp],
         "cfdg_mesh_cell_gid": [vp, vp],
         "cfdg_create": [vp, vp, vp, _i32, vp], "cfdg_destroy": [vp], "cfdg_sizes": [vp, vp, vp, vp],
@@ -339,8 +339,10 @@ class Mesh:
             raise CfdgError(rc, "cell_gid")
         return g
 
-    def reorder_rcm(self):
-        rc = lib.cfdg_mesh_reorder_rcm(self._h)
+    def reorder_rcm(self, on_device=False):
+        """Reverse Cuthill-McKee renumbering (cfdg_mesh_reorder_rcm), the
+        order computed on the host or on the device (same order)."""
+        rc = (lib.cfdg_mesh_reorder_rcm_device if on_device else lib.cfdg_mesh_reorder_rcm)(self._h)
         if rc != OK:
             raise CfdgError(rc, "reorder failed")
         lib.cfdg_mesh_get_view(self._h, C.byref(self.view))

@@ -307,6 +307,23 @@ int cfdg_mesh_geometry_device(cfdg_mesh* mesh) {
     }
 }
 
+int cfdg_mesh_reorder_rcm_device(cfdg_mesh* mesh) {
+    if (!mesh) return CFDG_ERR_ARG;
+    try {
+        cfdg::Mesh m = mesh->m;  // the mesh is left unchanged on error
+        cfdg::apply_cell_permutation(m, cfdg::rcm_order_device(m));
+        mesh->m = std::move(m);
+        mesh->conn.reset();
+        return CFDG_OK;
+    } catch (const cfdg::DeviceError& e) {
+        std::fprintf(stderr, "cfdg_mesh_reorder_rcm_device: %s\n", e.what());
+        return CFDG_ERR_HIP;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "cfdg_mesh_reorder_rcm_device: %s\n", e.what());
+        return CFDG_ERR_MESH;
+    }
+}
+
 int cfdg_mesh_connectivity(cfdg_mesh* mesh, int32_t on_device, int32_t node_stencil) {
     if (!mesh) return CFDG_ERR_ARG;
     try {

@@ -194,6 +194,139 @@ Csr csr_of(const Buf<u64>& keys, long long nkeys, int nrows) {
     return out;
 }
 
+
+// --- reverse Cuthill-McKee order (rcm_order, mesh.cpp; reorder.cpp:88-188) ---
+
+// Dual-graph keys: (cell, face) for both cells of every face whose other
+// cell is a cell of the mesh, (nc, 0) otherwise (sorted after every row).
+__global__ void __launch_bounds__(kBlock) k_dual_keys(int nf, int nc, const int* owner, const int* neigh, u64* keys) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int v = neigh[f];
+    const bool ok = v >= 0 && v < nc;
+    keys[2LL * f] = ok ? ((static_cast<u64>(owner[f]) << 32) | static_cast<unsigned>(f)) : (static_cast<u64>(nc) << 32);
+    keys[2LL * f + 1] = ok ? ((static_cast<u64>(v) << 32) | static_cast<unsigned>(f)) : (static_cast<u64>(nc) << 32);
+}
+
+// adjacency entry = the other cell of the entry's face
+__global__ void __launch_bounds__(kBlock) k_dual_adj(long long n, const u64* keys, const int* owner, const int* neigh,
+                                                     int* adj) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = static_cast<int>(keys[i] >> 32), f = static_cast<int>(keys[i] & 0xffffffffULL);
+    adj[i] = (owner[f] == c) ? neigh[f] : owner[f];
+}
+
+// Every unmarked neighbour v of the frontier queue[q0, q1) takes the smallest
+// key (queue position of the parent << 32 | adjacency slot) that reaches it:
+// the first parent in queue order and, within it, the first slot.
+__global__ void __launch_bounds__(kBlock) k_bfs_claim(int q0, int q1, const int* queue, const int* off, const int* adj,
+                                                      const unsigned char* mark, u64* claim) {
+    const int p = q0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= q1) return;
+    const int u = queue[p];
+    for (int e = off[u]; e < off[u + 1]; ++e) {
+        const int v = adj[e];
+        if (!mark[v]) atomicMin(&claim[v], (static_cast<u64>(p) << 32) | static_cast<unsigned>(e - off[u]));
+    }
+}
+
+// the winning slot of each claimed neighbour appends it (one winner per cell)
+__global__ void __launch_bounds__(kBlock) k_bfs_collect(int q0, int q1, const int* queue, const int* off, const int* adj,
+                                                        const unsigned char* mark, const u64* claim, u64* ckey,
+                                                        int* cval, int* count) {
+    const int p = q0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= q1) return;
+    const int u = queue[p];
+    for (int e = off[u]; e < off[u + 1]; ++e) {
+        const int v = adj[e];
+        const u64 key = (static_cast<u64>(p) << 32) | static_cast<unsigned>(e - off[u]);
+        if (!mark[v] && claim[v] == key) {
+            const int i = atomicAdd(count, 1);
+            ckey[i] = key;
+            cval[i] = v;
+        }
+    }
+}
+
+// keys (degree << 32 | cell) of the collected cells, the order among the
+// children of one parent in the Cuthill-McKee pass
+__global__ void __launch_bounds__(kBlock) k_deg_keys(int n, const int* cval, const int* off, u64* ckey) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = cval[i];
+    ckey[i] = (static_cast<u64>(static_cast<unsigned>(off[v + 1] - off[v])) << 32) | static_cast<unsigned>(v);
+}
+
+// keys = parent position of the collected cells
+__global__ void __launch_bounds__(kBlock) k_parent_keys(int n, const int* cval, const u64* claim, u64* ckey) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ckey[i] = claim[cval[i]] >> 32;
+}
+
+// the sorted level goes to queue[q1, q1 + n), marked, its claims cleared
+__global__ void __launch_bounds__(kBlock) k_bfs_commit(int n, const int* cval, int* queue_out, unsigned char* mark,
+                                                       u64* claim) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = cval[i];
+    queue_out[i] = v;
+    mark[v] = 1;
+    claim[v] = ~0ULL;
+}
+
+__global__ void __launch_bounds__(kBlock) k_first_unmarked(int n, const unsigned char* mark, int* first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !mark[i]) atomicMin(first, i);
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u64(long long n, u64 v, u64* a) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_set_one(int v, int pos, int* queue, unsigned char* mark) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[pos] = v;
+        mark[v] = 1;
+    }
+}
+
+// Device scratch that grows on demand and is reused across calls.
+struct Scratch {
+    void* p = nullptr;
+    std::size_t n = 0;
+    Scratch() = default;
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    ~Scratch() {
+        if (p) (void)hipFree(p);
+    }
+    void* get(std::size_t want) {
+        if (want > n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+            hip_ok(hipMalloc(&p, want), "hipMalloc");
+            n = want;
+        }
+        return p;
+    }
+};
+
+// pairs sorted by key over bits [0, end_bit), stable
+void sort_pairs(Buf<u64>& k, Buf<u64>& k_alt, Buf<int>& v, Buf<int>& v_alt, long long n, int end_bit, Scratch& tmp) {
+    if (n <= 1) return;
+    hipcub::DoubleBuffer<u64> dk(k.p, k_alt.p);
+    hipcub::DoubleBuffer<int> dv(v.p, v_alt.p);
+    std::size_t bytes = 0;
+    hip_ok(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, dk, dv, n, 0, end_bit), "radix sort size");
+    hip_ok(hipcub::DeviceRadixSort::SortPairs(tmp.get(bytes), bytes, dk, dv, n, 0, end_bit), "radix sort");
+    if (dk.Current() != k.p) std::swap(k.p, k_alt.p);
+    if (dv.Current() != v.p) std::swap(v.p, v_alt.p);
+}
+
 }  // namespace
 
 MeshConnectivity device_connectivity(const Mesh& m, bool node_stencil) {
@@ -318,6 +451,131 @@ MeshConnectivity device_connectivity(const Mesh& m, bool node_stencil) {
         for (int r = 0; r < n; ++r) st.off[static_cast<std::size_t>(lo + r) + 1] = static_cast<int>(base + lo_off[static_cast<std::size_t>(r) + 1]);
         lo = hi;
     }
+    return out;
+}
+
+std::vector<std::int32_t> rcm_order_device(const Mesh& m) {
+    const int n = m.n_cells, nf = m.n_faces;
+    if (n <= 0 || nf <= 0) throw std::runtime_error("device rcm: mesh sizes");
+    // dual graph: each cell's neighbours in ascending face order, as the host builds it
+    Buf<int> owner(m.face_owner), neigh(m.face_neigh);
+    Buf<int> off(static_cast<std::size_t>(n) + 1);
+    long long ne = 0;
+    for (int f = 0; f < nf; ++f) {
+        const int v = m.face_neigh[static_cast<std::size_t>(f)];
+        if (v >= 0 && v < n) ne += 2;
+    }
+    Buf<int> adj(static_cast<std::size_t>(std::max<long long>(ne, 1)));
+    {
+        Buf<u64> keys(2 * static_cast<std::size_t>(nf)), alt(2 * static_cast<std::size_t>(nf));
+        k_dual_keys<<<grid(nf), kBlock>>>(nf, n, owner.p, neigh.p, keys.p);
+        hip_ok(hipGetLastError(), "launch");
+        sort_keys(keys, alt, 2LL * nf, 32 + bit_width(n));
+        k_row_offsets<<<grid(n + 1LL), kBlock>>>(ne, keys.p, n, off.p);
+        if (ne > 0) k_dual_adj<<<grid(ne), kBlock>>>(ne, keys.p, owner.p, neigh.p, adj.p);
+        hip_ok(hipGetLastError(), "launch");
+    }
+    std::vector<int> h_off(static_cast<std::size_t>(n) + 1);
+    off.get(h_off.data(), h_off.size());
+    const auto deg = [&](int v) { return h_off[static_cast<std::size_t>(v) + 1] - h_off[static_cast<std::size_t>(v)]; };
+
+    Buf<unsigned char> visited(static_cast<std::size_t>(n)), mark(static_cast<std::size_t>(n));
+    hip_ok(hipMemset(visited.p, 0, static_cast<std::size_t>(n)), "memset");
+    Buf<u64> claim(static_cast<std::size_t>(n));
+    k_fill_u64<<<grid(n), kBlock>>>(n, ~0ULL, claim.p);
+    Buf<int> pq(static_cast<std::size_t>(n)), order(static_cast<std::size_t>(n)), cnt(1), first(1);
+    Buf<u64> ckey(static_cast<std::size_t>(n)), ckey_alt(static_cast<std::size_t>(n));
+    Buf<int> cval(static_cast<std::size_t>(n)), cval_alt(static_cast<std::size_t>(n));
+    Scratch tmp;
+    hip_ok(hipGetLastError(), "launch");
+
+    // one BFS level: the frontier queue[q0, q1) appends its unmarked
+    // neighbours at q1 in the order of the mode (0: parent, then adjacency
+    // slot, as a plain queue; 1: parent, then degree, then index); returns
+    // how many
+    const auto level = [&](int* queue, int q0, int q1, int mode) -> int {
+        hip_ok(hipMemset(cnt.p, 0, sizeof(int)), "memset");
+        k_bfs_claim<<<grid(q1 - q0), kBlock>>>(q0, q1, queue, off.p, adj.p, mark.p, claim.p);
+        k_bfs_collect<<<grid(q1 - q0), kBlock>>>(q0, q1, queue, off.p, adj.p, mark.p, claim.p, ckey.p, cval.p, cnt.p);
+        hip_ok(hipGetLastError(), "launch");
+        int c = 0;
+        cnt.get(&c, 1);
+        if (c < 0 || static_cast<long long>(q1) + c > n) throw DeviceError("device rcm: level exceeds the cell count");
+        if (c == 0) return 0;
+        const int pbits = bit_width(q1);
+        if (mode == 0) {
+            sort_pairs(ckey, ckey_alt, cval, cval_alt, c, 32 + pbits, tmp);
+        } else {
+            k_deg_keys<<<grid(c), kBlock>>>(c, cval.p, off.p, ckey.p);
+            hip_ok(hipGetLastError(), "launch");
+            sort_pairs(ckey, ckey_alt, cval, cval_alt, c, 64, tmp);
+            k_parent_keys<<<grid(c), kBlock>>>(c, cval.p, claim.p, ckey.p);
+            hip_ok(hipGetLastError(), "launch");
+            sort_pairs(ckey, ckey_alt, cval, cval_alt, c, pbits, tmp);
+        }
+        k_bfs_commit<<<grid(c), kBlock>>>(c, cval.p, queue + q1, mark.p, claim.p);
+        hip_ok(hipGetLastError(), "launch");
+        return c;
+    };
+
+    // pseudo-peripheral start: up to three BFS passes from the deepest cell of
+    // the previous one (lowest degree among the deepest, first in queue order)
+    const auto peripheral = [&](int root) {
+        int best = root, max_depth = 0;
+        std::vector<int> last;
+        for (int pass = 0; pass < 3; ++pass) {
+            hip_ok(hipMemcpy(mark.p, visited.p, static_cast<std::size_t>(n), hipMemcpyDeviceToDevice), "copy");
+            k_set_one<<<1, 1>>>(best, 0, pq.p, mark.p);
+            hip_ok(hipGetLastError(), "launch");
+            int q0 = 0, q1 = 1, depth = 0;
+            for (;;) {
+                const int c = level(pq.p, q0, q1, 0);
+                if (c == 0) break;
+                q0 = q1;
+                q1 += c;
+                ++depth;
+            }
+            int far = best;
+            if (depth > 0) {
+                last.resize(static_cast<std::size_t>(q1 - q0));
+                hip_ok(hipMemcpy(last.data(), pq.p + q0, last.size() * sizeof(int), hipMemcpyDeviceToHost), "copy");
+                far = last[0];
+                for (const int v : last)
+                    if (deg(v) < deg(far)) far = v;
+            }
+            if (depth <= max_depth) break;
+            max_depth = depth;
+            best = far;
+        }
+        return best;
+    };
+
+    int total = 0;
+    while (total < n) {
+        int f = n;
+        hip_ok(hipMemcpy(first.p, &f, sizeof(int), hipMemcpyHostToDevice), "copy");
+        k_first_unmarked<<<grid(n), kBlock>>>(n, visited.p, first.p);
+        hip_ok(hipGetLastError(), "launch");
+        first.get(&f, 1);
+        if (f >= n) throw DeviceError("device rcm: no unvisited cell left before the order is complete");
+        const int start = peripheral(f);
+        // Cuthill-McKee pass of this component, marks kept in `visited`
+        hip_ok(hipMemcpy(mark.p, visited.p, static_cast<std::size_t>(n), hipMemcpyDeviceToDevice), "copy");
+        k_set_one<<<1, 1>>>(start, total, order.p, mark.p);
+        hip_ok(hipGetLastError(), "launch");
+        int q0 = total, q1 = total + 1;
+        for (;;) {
+            const int c = level(order.p, q0, q1, 1);
+            if (c == 0) break;
+            q0 = q1;
+            q1 += c;
+        }
+        hip_ok(hipMemcpy(visited.p, mark.p, static_cast<std::size_t>(n), hipMemcpyDeviceToDevice), "copy");
+        total = q1;
+    }
+    std::vector<std::int32_t> out(static_cast<std::size_t>(n));
+    order.get(out.data(), out.size());
+    std::reverse(out.begin(), out.end());
     return out;
 }
 

@@ -63,6 +63,10 @@ struct DeviceError : std::runtime_error {
 // to host_connectivity's entry for entry; throws DeviceError on a HIP failure
 // and std::runtime_error on a mesh whose faces couple a pair of cells twice.
 MeshConnectivity device_connectivity(const Mesh& m, bool node_stencil);
+// The reverse Cuthill-McKee order of rcm_order (mesh.cpp) computed on the
+// current HIP device (kernels_conn.hip): the same new -> old permutation,
+// entry for entry; throws DeviceError on a HIP failure.
+std::vector<std::int32_t> rcm_order_device(const Mesh& m);
 
 // Row groups for the parallel Gauss-Seidel sweeps: rows of one group have no
 // matrix coupling, so they can be updated together.

@@ -166,6 +166,13 @@ int cfdg_mesh_cell_gid(const cfdg_mesh* mesh, int64_t* gid);
 /* Renumbers cells in reverse Cuthill-McKee order and re-sorts faces
    (src/mesh/reorder.cpp:135-188, 306-576). */
 int cfdg_mesh_reorder_rcm(cfdg_mesh* mesh);
+/* The same renumbering with the order computed on the current HIP device
+   (level-synchronous breadth-first passes whose ties are broken as the
+   sequential ones break them: pseudo-peripheral start, then children by
+   (degree, index) under the first parent in queue order), giving the same
+   cell order entry for entry.  CFDG_ERR_HIP on a device failure, the mesh
+   then left unchanged. */
+int cfdg_mesh_reorder_rcm_device(cfdg_mesh* mesh);
 /* Recomputes the cell centroids and volumes and the face centroids, unit
    normals and areas from the nodes and node lists on the device
    (compute_mesh_geometry, src/mesh/geometry.cpp:301-539), with the same
